@@ -1,0 +1,131 @@
+/*
+ * sinv.h -- C ABI of libsinv_hip, the MI355X (gfx950) selected-inversion hot
+ * path for block-tridiagonal matrices.
+ *
+ * The reference (Nano-TCAD/SINV) is pure Python/numpy and has no C layer; each
+ * entry point below names the reference function whose arithmetic it replaces.
+ * Callers are the Python host layer (sinv_amd, via ctypes) or any C caller.
+ *
+ * Conventions
+ *   - complex128 = two interleaved doubles (numpy/torch layout); blocks are
+ *     row-major b x b; stacked layout diag (N,b,b), upper[i] = A_{i,i+1},
+ *     lower[i] = A_{i+1,i} (src/sinv/utils/matrices_utils.py:141-151).
+ *   - All pointers except descriptors and block-size tables are DEVICE
+ *     pointers.  `stream` is a hipStream_t (NULL = default stream).
+ *   - Nothing here allocates: the caller passes a workspace whose size comes
+ *     from the matching *_workspace_bytes query.  Calls are asynchronous and
+ *     stream-ordered; `info` is written on the device.
+ *   - Return codes: SINV_OK, SINV_E_INVAL (bad arguments, nothing launched),
+ *     SINV_E_HIP (a HIP runtime error; text via sinv_last_error()).
+ *     Singular pivots are reported through `info` (device), not the return
+ *     code, because the call does not synchronise: info holds
+ *     (block << 32 | instance) of the first singular leaf pivot, or
+ *     SINV_INFO_OK.
+ */
+#ifndef SINV_H_
+#define SINV_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SINV_OK 0
+#define SINV_E_SINGULAR 1
+#define SINV_E_INVAL 2
+#define SINV_E_HIP 3
+
+#define SINV_INFO_OK (~0ULL)
+
+#define SINV_PREC_FP64 0  /* complex128 GEMMs on FP64 MFMA, FP64 inverses   */
+#define SINV_PREC_MIXED 1 /* complex64 GEMMs on FP32 MFMA, FP64 inverses    */
+
+#define SINV_OP_N 0
+#define SINV_OP_T 1
+#define SINV_OP_C 2
+
+typedef struct sinv_c128 {
+  double re, im;
+} sinv_c128;
+
+/* Library version, major*10000 + minor*100 + patch. */
+int sinv_version(void);
+/* Text of the last error on the calling thread ("" if none). */
+const char* sinv_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * RGF selected inversion, batched over instances.
+ * Replaces rgf.rgf (src/sinv/algorithms/rgf/rgf.py:17-64): forward sweep
+ * rgf.py:43-48, backward sweep rgf.py:50-62, `sym_mat` / `save_off_diag`
+ * keyword semantics of rgf.py:18-20 (off-diagonals NaN when not saved,
+ * rgf.py:41,55; plain transpose when sym_mat, rgf.py:58).
+ *
+ * Instance z has blocks
+ *   A_ii      = dscale * diag[z][i] + shift[z] * I
+ *   A_{i,i+1} = oscale * upper[z][i],  A_{i+1,i} = oscale * lower[z][i]
+ * so A(E) = (E + i eta) I - H is dscale = oscale = -1, shift = E + i eta with
+ * H shared (stride 0).  Plain input A is dscale = oscale = 1, shift = NULL.
+ *
+ * Block sizes: uniform `blocksize` when `bsizes` is NULL; otherwise bsizes
+ * (host, nblocks entries) gives b_i and blocks are packed back to back:
+ * diag block i at sum_{j<i} b_j^2, upper/lower block i (b_i x b_{i+1} /
+ * b_{i+1} x b_i) at sum_{j<i} b_j b_{j+1}.  The *_stride fields are the
+ * element distance between instances (0 = shared by all instances).
+ * Outputs g_* are instance-major with the packed per-instance size.
+ * ------------------------------------------------------------------------ */
+typedef struct sinv_rgf_desc {
+  int nblocks;
+  int blocksize;
+  const int* bsizes;
+  int batch;
+  int sym_mat;
+  int save_off_diag;
+  int precision;
+  const void* diag;  long long diag_stride;
+  const void* upper; long long upper_stride;
+  const void* lower; long long lower_stride;
+  sinv_c128 dscale;
+  sinv_c128 oscale;
+  const void* shift;
+  void* g_diag;
+  void* g_upper;
+  void* g_lower;
+  unsigned long long* info;
+} sinv_rgf_desc;
+
+size_t sinv_rgf_workspace_bytes(const sinv_rgf_desc* d);
+int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Batched in-place complex128 block inverse (kernel K1).
+ * Replaces the numpy.linalg.inv calls of the hot path (rgf.py:44,48;
+ * bcr_serial.py:121,148,249,302,379).  Blocks with n <= 64 use Gauss-Jordan
+ * with partial pivoting (the pivoting of LAPACK ?gesv); larger blocks use a
+ * 2x2 Schur-complement recursion down to such leaves.
+ * Matrix z is at A + z*strideA (complex elements), leading dimension lda.
+ * ------------------------------------------------------------------------ */
+size_t sinv_zinv_workspace_bytes(int n, int batch);
+int sinv_zinv_batched(void* A, int n, int lda, long long strideA, int batch, void* work,
+                      size_t work_bytes, unsigned long long* info, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Batched complex GEMM (kernel K2):
+ *   C[z] = alpha * op(A[z]) op(B[z]) + beta * C0[z] (+ shift[z] * I)
+ * Replaces the ndarray `@` products of the hot path (rgf.py:48,53,60,62;
+ * bcr_serial.py:124-169, 293-304, 354-383).  precision = SINV_PREC_FP64
+ * (FP64 MFMA) or SINV_PREC_MIXED (operands rounded to complex64, FP32 MFMA,
+ * FP32 accumulation).  C0 may be NULL (beta ignored) or alias C.
+ * ------------------------------------------------------------------------ */
+int sinv_zgemm_batched(int M, int N, int K, int opA, int opB, sinv_c128 alpha,
+                       const void* A, int lda, long long strideA,
+                       const void* B, int ldb, long long strideB,
+                       sinv_c128 beta, const void* C0, int ldc0, long long strideC0,
+                       void* C, int ldc, long long strideC,
+                       const void* shift, int batch, int precision, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SINV_H_ */

@@ -1,0 +1,147 @@
+"""ctypes binding of libsinv_hip.so (the C ABI declared in include/sinv.h).
+
+The library is loaded after torch so that both use the HIP runtime bundled
+with torch (the library's run path points there as well).  There is no CPU
+fallback: if the library is missing, every entry point raises.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+SINV_OK = 0
+SINV_E_SINGULAR = 1
+SINV_E_INVAL = 2
+SINV_E_HIP = 3
+SINV_INFO_OK = (1 << 64) - 1
+
+PREC = {"fp64": 0, "mixed": 1}
+OPS = {"N": 0, "T": 1, "C": 2}
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsinv_hip.so")
+
+class C128(ctypes.Structure):
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+    @classmethod
+    def of(cls, z):
+        z = complex(z)
+        return cls(z.real, z.imag)
+
+
+_vp = ctypes.c_void_p
+_ll = ctypes.c_longlong
+_i = ctypes.c_int
+
+
+class RgfDesc(ctypes.Structure):
+    _fields_ = [
+        ("nblocks", _i), ("blocksize", _i), ("bsizes", ctypes.POINTER(_i)), ("batch", _i),
+        ("sym_mat", _i), ("save_off_diag", _i), ("precision", _i),
+        ("diag", _vp), ("diag_stride", _ll),
+        ("upper", _vp), ("upper_stride", _ll),
+        ("lower", _vp), ("lower_stride", _ll),
+        ("dscale", C128), ("oscale", C128),
+        ("shift", _vp),
+        ("g_diag", _vp), ("g_upper", _vp), ("g_lower", _vp),
+        ("info", ctypes.POINTER(ctypes.c_ulonglong)),
+    ]
+
+
+class BcrDesc(ctypes.Structure):
+    _fields_ = [
+        ("nblocks", _i), ("blocksize", _i), ("batch", _i),
+        ("diag", _vp), ("diag_stride", _ll),
+        ("upper", _vp), ("upper_stride", _ll),
+        ("lower", _vp), ("lower_stride", _ll),
+        ("dscale", C128), ("oscale", C128),
+        ("shift", _vp),
+        ("g_diag", _vp), ("g_upper", _vp), ("g_lower", _vp),
+        ("g_fill", _vp),
+        ("info", ctypes.POINTER(ctypes.c_ulonglong)),
+    ]
+
+
+class AdjDesc(ctypes.Structure):
+    _fields_ = [
+        ("nblocks", _i), ("blocksize", _i), ("batch", _i), ("precision", _i),
+        ("checkpoint_every", _i),
+        ("diag", _vp), ("diag_stride", _ll),
+        ("upper", _vp), ("upper_stride", _ll),
+        ("lower", _vp), ("lower_stride", _ll),
+        ("dscale", C128), ("oscale", C128),
+        ("shift", _vp),
+        ("gbar_diag", _vp), ("gbar_diag_stride", _ll),
+        ("gbar_upper", _vp), ("gbar_upper_stride", _ll),
+        ("gbar_lower", _vp), ("gbar_lower_stride", _ll),
+        ("g_diag", _vp), ("g_upper", _vp), ("g_lower", _vp),
+        ("abar_diag", _vp), ("abar_upper", _vp), ("abar_lower", _vp),
+        ("accumulate", _i),
+        ("info", ctypes.POINTER(ctypes.c_ulonglong)),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libsinv_hip.so (raises ImportError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m sinv_amd.build` (there is no CPU fallback)")
+    import torch  # noqa: F401  -- load torch's HIP runtime first
+    L = ctypes.CDLL(LIB_PATH)
+    L.sinv_version.restype = _i
+    L.sinv_last_error.restype = ctypes.c_char_p
+    L.sinv_rgf_workspace_bytes.restype = ctypes.c_size_t
+    L.sinv_rgf_workspace_bytes.argtypes = [ctypes.POINTER(RgfDesc)]
+    L.sinv_rgf_batched.restype = _i
+    L.sinv_rgf_batched.argtypes = [ctypes.POINTER(RgfDesc), _vp, ctypes.c_size_t, _vp]
+    L.sinv_zinv_workspace_bytes.restype = ctypes.c_size_t
+    L.sinv_zinv_workspace_bytes.argtypes = [_i, _i]
+    L.sinv_zinv_batched.restype = _i
+    L.sinv_zinv_batched.argtypes = [_vp, _i, _i, _ll, _i, _vp, ctypes.c_size_t,
+                                    ctypes.POINTER(ctypes.c_ulonglong), _vp]
+    L.sinv_zgemm_batched.restype = _i
+    L.sinv_zgemm_batched.argtypes = [_i, _i, _i, _i, _i, C128, _vp, _i, _ll, _vp, _i, _ll,
+                                     C128, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _i, _vp]
+    if hasattr(L, "sinv_bcr_batched"):
+        L.sinv_bcr_workspace_bytes.restype = ctypes.c_size_t
+        L.sinv_bcr_workspace_bytes.argtypes = [ctypes.POINTER(BcrDesc)]
+        L.sinv_bcr_batched.restype = _i
+        L.sinv_bcr_batched.argtypes = [ctypes.POINTER(BcrDesc), _vp, ctypes.c_size_t, _vp]
+    if hasattr(L, "sinv_rgf_adjoint_batched"):
+        L.sinv_rgf_adjoint_workspace_bytes.restype = ctypes.c_size_t
+        L.sinv_rgf_adjoint_workspace_bytes.argtypes = [ctypes.POINTER(AdjDesc)]
+        L.sinv_rgf_adjoint_batched.restype = _i
+        L.sinv_rgf_adjoint_batched.argtypes = [ctypes.POINTER(AdjDesc), _vp, ctypes.c_size_t, _vp]
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    """Map a C return code to the reference's exception types."""
+    if rc == SINV_OK:
+        return
+    msg = lib().sinv_last_error().decode(errors="replace")
+    if rc == SINV_E_INVAL:
+        raise ValueError(f"{what}: {msg}")
+    if rc == SINV_E_SINGULAR:
+        raise np.linalg.LinAlgError("Singular matrix")
+    raise RuntimeError(f"{what}: HIP error: {msg}")
+
+
+def check_info(info_tensor, what="inverse"):
+    """Raise numpy's LinAlgError (as numpy.linalg.inv does) if a pivot was zero.
+
+    Synchronises the stream of ``info_tensor``."""
+    v = int(info_tensor.cpu().numpy().view(np.uint64)[0])
+    if v != SINV_INFO_OK:
+        block, inst = v >> 32, v & 0xFFFFFFFF
+        err = np.linalg.LinAlgError("Singular matrix")
+        err.block, err.instance = block, inst
+        raise err

@@ -1,0 +1,180 @@
+"""Device-side entry points of the selected-inversion hot path.
+
+Every function here runs on the GPU through libsinv_hip (ctypes).  Inputs may
+be numpy arrays (copied to the device) or torch tensors (used in place when
+already complex128, contiguous and on the GPU); outputs are torch tensors on
+the device.  Calls are ordered on torch's current stream.
+
+Block layout (matrices_utils.py:141-151): diag (N,b,b), upper[i] = A_{i,i+1},
+lower[i] = A_{i+1,i}; a leading batch dimension (B, ...) gives one instance
+per batch entry, and a missing one means the blocks are shared by every
+instance.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+_c128 = torch.complex128
+
+
+def device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("sinv_amd runs on a ROCm GPU; no GPU is visible (there is no CPU fallback)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def to_dev(x):
+    """numpy / torch -> contiguous complex128 tensor on the current GPU."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        x = torch.from_numpy(np.ascontiguousarray(x))
+    elif not isinstance(x, torch.Tensor):
+        x = torch.as_tensor(x)
+    return x.to(device=device(), dtype=_c128).contiguous()
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _info_ptr(info):
+    return ctypes.cast(ctypes.c_void_p(info.data_ptr()), ctypes.POINTER(ctypes.c_ulonglong))
+
+
+def _workspace(nbytes):
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device())
+
+
+def _batched(x, nd):
+    """Return (tensor, batch or None, element stride between instances)."""
+    if x.dim() == nd + 1:
+        return x, x.shape[0], x[0].numel() if x.shape[0] > 0 else 0
+    if x.dim() == nd:
+        return x, None, 0
+    raise ValueError(f"expected a {nd}-d block stack or a batch of them, got shape {tuple(x.shape)}")
+
+
+def _resolve_batch(*cands):
+    bs = {c for c in cands if c is not None}
+    if len(bs) > 1:
+        raise ValueError(f"inconsistent batch sizes {sorted(bs)}")
+    return bs.pop() if bs else 1
+
+
+def rgf_batched(diag, upper, lower, *, sym_mat=False, save_off_diag=True, precision="fp64",
+                shift=None, dscale=1.0, oscale=1.0, check=True):
+    """RGF selected inversion of a batch of block-tridiagonal instances.
+
+    Instance z is A_ii = dscale*diag[z][i] + shift[z]*I, A_{i,i+1} = oscale*upper[z][i],
+    A_{i+1,i} = oscale*lower[z][i]; blocks without a batch dimension are shared.
+    Returns (G_diag (B,N,b,b), G_upper (B,N-1,b,b), G_lower (B,N-1,b,b)).
+    Follows rgf.py:40-62; with save_off_diag=False the off-diagonals are NaN
+    (rgf.py:41,55), with sym_mat=True G_{i,i+1} = G_{i+1,i}^T (rgf.py:58).
+    """
+    if precision not in nat.PREC:
+        raise ValueError(f"precision must be one of {sorted(nat.PREC)}")
+    d, Bd, sd = _batched(to_dev(diag), 3)
+    N, b = d.shape[-3], d.shape[-1]
+    if d.shape[-2] != b:
+        raise ValueError("diagonal blocks must be square")
+    if N == 0:
+        raise ValueError("need at least one block")
+    if N > 1:
+        u, Bu, su = _batched(to_dev(upper), 3)
+        l, Bl, sl = _batched(to_dev(lower), 3)
+        if u.shape[-3:] != (N - 1, b, b) or l.shape[-3:] != (N - 1, b, b):
+            raise ValueError("upper/lower must hold N-1 blocks of the diagonal block size")
+    else:
+        u = l = None
+        Bu = Bl = None
+        su = sl = 0
+    sh = None
+    Bs = None
+    if shift is not None:
+        sh = to_dev(shift).reshape(-1)
+        Bs = sh.numel()
+    B = _resolve_batch(Bd, Bu, Bl, Bs)
+    dev = device()
+    Gd = torch.empty((B, N, b, b), dtype=_c128, device=dev)
+    Gu = torch.empty((B, max(N - 1, 0), b, b), dtype=_c128, device=dev)
+    Gl = torch.empty((B, max(N - 1, 0), b, b), dtype=_c128, device=dev)
+    info = torch.empty(1, dtype=torch.int64, device=dev)
+    desc = nat.RgfDesc(
+        nblocks=N, blocksize=b, bsizes=None, batch=B, sym_mat=int(bool(sym_mat)),
+        save_off_diag=int(bool(save_off_diag)), precision=nat.PREC[precision],
+        diag=_p(d), diag_stride=sd, upper=_p(u), upper_stride=su, lower=_p(l), lower_stride=sl,
+        dscale=nat.C128.of(dscale), oscale=nat.C128.of(oscale), shift=_p(sh),
+        g_diag=_p(Gd), g_upper=_p(Gu), g_lower=_p(Gl), info=_info_ptr(info))
+    L = nat.lib()
+    ws = _workspace(L.sinv_rgf_workspace_bytes(ctypes.byref(desc)))
+    nat.check(L.sinv_rgf_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()), "rgf")
+    if check:
+        nat.check_info(info)
+    return Gd, Gu, Gl
+
+
+def rgf_energies(H_diag, H_upper, H_lower, energies, eta, **kw):
+    """RGF of A(E_k) = (E_k + i*eta) I - H for every energy, H shared.
+
+    Only H is stored; A(E_k) is formed inside the kernels' epilogues."""
+    e = torch.as_tensor(np.asarray(energies), dtype=torch.complex128) + 1j * float(eta)
+    return rgf_batched(H_diag, H_upper, H_lower, shift=e, dscale=-1.0, oscale=-1.0, **kw)
+
+
+def zinv_batched(A, check=True):
+    """In-place-style batched inverse of (..., n, n) complex128 blocks; returns a new tensor."""
+    X = to_dev(A).clone()
+    n = X.shape[-1]
+    batch = int(np.prod(X.shape[:-2])) if X.dim() > 2 else 1
+    info = torch.empty(1, dtype=torch.int64, device=X.device)
+    L = nat.lib()
+    ws = _workspace(L.sinv_zinv_workspace_bytes(n, batch))
+    nat.check(L.sinv_zinv_batched(_p(X), n, n, n * n, batch, _p(ws), ws.numel(), _info_ptr(info),
+                                  _stream()), "zinv")
+    if check:
+        nat.check_info(info)
+    return X
+
+
+def zgemm_batched(A, B, *, opA="N", opB="N", alpha=1.0, beta=0.0, C0=None, shift=None,
+                  precision="fp64"):
+    """C = alpha op(A) op(B) + beta C0 (+ shift I) over a batch (leading dim) of blocks."""
+    A = to_dev(A)
+    B = to_dev(B)
+    if A.dim() == 2:
+        A = A[None]
+    if B.dim() == 2:
+        B = B[None]
+    batch = max(A.shape[0], B.shape[0])
+    ar, ac = A.shape[-2:]
+    br, bc = B.shape[-2:]
+    M, K = (ar, ac) if opA == "N" else (ac, ar)
+    K2, N = (br, bc) if opB == "N" else (bc, br)
+    if K != K2:
+        raise ValueError("inner dimensions differ")
+    sA = A[0].numel() if A.shape[0] > 1 else 0
+    sB = B[0].numel() if B.shape[0] > 1 else 0
+    C = torch.empty((batch, M, N), dtype=_c128, device=A.device)
+    if C0 is not None:
+        C0 = to_dev(C0)
+        if C0.dim() == 2:
+            C0 = C0[None]
+        sC0 = C0[0].numel() if C0.shape[0] > 1 else 0
+    else:
+        sC0 = 0
+    sh = to_dev(shift).reshape(-1) if shift is not None else None
+    L = nat.lib()
+    nat.check(L.sinv_zgemm_batched(
+        M, N, K, nat.OPS[opA], nat.OPS[opB], nat.C128.of(alpha), _p(A), ac, sA, _p(B), bc, sB,
+        nat.C128.of(beta), _p(C0), N, sC0, _p(C), N, M * N, _p(sh), batch, nat.PREC[precision],
+        _stream()), "zgemm")
+    return C

@@ -1,0 +1,417 @@
+// K2: batched complex GEMM on CDNA4 matrix cores, plus small elementwise kernels.
+//
+// C[z] = alpha * op(A[z]) op(B[z]) + beta * C0[z] + shift[z] * I
+//
+// Design (gfx950, wave64):
+//  * workgroup = 256 threads = 4 waves in a 2 x 2 arrangement; workgroup tile
+//    64 x 64 complex, wave tile 32 x 32 = 2 x 2 MFMA tiles of 16 x 16;
+//  * K advances in chunks of 16 complex staged through LDS, with the next
+//    chunk's global loads issued into registers before the current chunk's
+//    MFMAs (single LDS buffer, register prefetch);
+//  * a complex product is 4 real MFMAs (re*re, -im*im, re*im, im*re) on
+//    v_mfma_f64_16x16x4_f64 (FP64) or v_mfma_f32_16x16x4_f32 (complex64 mode,
+//    operands rounded to float when staged into LDS, FP32 accumulation);
+//  * op(A) and op(B) tiles are written to LDS k-major, so each MFMA operand
+//    read is 16 consecutive complex per quarter-wave;
+//  * the linear workgroup id is remapped so that consecutive tiles (same GEMM,
+//    neighbouring batch entries) land on the same XCD and share its L2.
+//
+// MFMA 16x16x4 operand layout (lane l of 64): A[i][k] with i = l % 16,
+// k = l / 16; B[k][j] with j = l % 16, k = l / 16.  Accumulator item v holds
+// D[l/16 + 4*v][l % 16] for the FP64 instruction and D[4*(l/16) + v][l % 16]
+// for the FP32 one (measured on gfx950 with lane-coded operands).
+#include <cstdarg>
+#include <cstdio>
+
+#include "sinv_core.hpp"
+
+namespace sinv {
+
+namespace {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <typename R>
+struct CR {
+  R re, im;
+};
+
+template <typename R>
+struct Acc;
+template <>
+struct Acc<double> {
+  using V = d4v;
+};
+template <>
+struct Acc<float> {
+  using V = f4v;
+};
+
+__device__ __forceinline__ d4v mfma(double a, double b, d4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4v mfma(float a, float b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// row of accumulator item v in lane l, within a 16 x 16 tile
+template <typename R>
+__device__ __forceinline__ int acc_row(int lane, int v);
+template <>
+__device__ __forceinline__ int acc_row<double>(int lane, int v) {
+  return (lane >> 4) + 4 * v;
+}
+template <>
+__device__ __forceinline__ int acc_row<float>(int lane, int v) {
+  return (lane >> 4) * 4 + v;
+}
+
+constexpr int BM = 64;
+constexpr int BN = 64;
+constexpr int BK = 16;
+constexpr int NTHR = 256;
+constexpr int PAD = 1;
+
+struct TileMap {
+  int z_in, z_out, m0, n0;
+};
+
+// XCD-aware remap: hardware places workgroup `bid` on XCD bid % 8.  Give each
+// XCD a contiguous range of logical tiles.
+__device__ __forceinline__ TileMap map_tile(int tiles_m, int tiles_n, int inner) {
+  const int total = gridDim.x;
+  int bid = blockIdx.x;
+  if ((total & 7) == 0) {
+    const int per = total >> 3;
+    bid = (bid & 7) * per + (bid >> 3);
+  }
+  const int tiles = tiles_m * tiles_n;
+  const int z = bid / tiles;
+  const int t = bid - z * tiles;
+  TileMap m;
+  m.z_out = z / inner;
+  m.z_in = z - m.z_out * inner;
+  m.m0 = (t / tiles_n) * BM;
+  m.n0 = (t % tiles_n) * BN;
+  return m;
+}
+
+__device__ __forceinline__ cplx ld_c(const cplx* p) {
+  const double2 v = *reinterpret_cast<const double2*>(p);
+  return cplx{v.x, v.y};
+}
+
+template <typename R>
+__global__ __launch_bounds__(NTHR) void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
+  __shared__ CR<R> As[BK][BM + PAD];
+  __shared__ CR<R> Bs[BK][BN + PAD];
+
+  const TileMap tm = map_tile(tiles_m, tiles_n, g.inner);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  const cplx* A = g.A + tm.z_in * g.sA_in + tm.z_out * g.sA_out;
+  const cplx* B = g.B + tm.z_in * g.sB_in + tm.z_out * g.sB_out;
+
+  // per-thread global-load coordinates (4 consecutive complex each)
+  // op(A) tile is BM x BK; op(B) tile is BK x BN.
+  const bool a_kcontig = (g.opA == OP_N);  // A row-major M x K: contiguous along k
+  const bool b_ncontig = (g.opB == OP_N);  // B row-major K x N: contiguous along n
+  const double a_conj = (g.opA == OP_C) ? -1.0 : 1.0;
+  const double b_conj = (g.opB == OP_C) ? -1.0 : 1.0;
+
+  int a_r, a_c;  // (m, k) of first element in tile coordinates
+  if (a_kcontig) { a_r = tid >> 2; a_c = (tid & 3) * 4; }
+  else           { a_c = tid >> 4; a_r = (tid & 15) * 4; }
+  int b_r, b_c;  // (k, n)
+  if (b_ncontig) { b_r = tid >> 4; b_c = (tid & 15) * 4; }
+  else           { b_c = tid >> 2; b_r = (tid & 3) * 4; }
+
+  cplx ra[4], rb[4];
+
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      int m = tm.m0 + a_r + (a_kcontig ? 0 : e);
+      int k = k0 + a_c + (a_kcontig ? e : 0);
+      cplx v{0.0, 0.0};
+      if (m < g.M && k < g.K) {
+        v = a_kcontig ? ld_c(A + (long long)m * g.lda + k) : ld_c(A + (long long)k * g.lda + m);
+        v.im *= a_conj;
+      }
+      ra[e] = v;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      int k = k0 + b_r + (b_ncontig ? 0 : e);
+      int n = tm.n0 + b_c + (b_ncontig ? e : 0);
+      cplx v{0.0, 0.0};
+      if (k < g.K && n < g.N) {
+        v = b_ncontig ? ld_c(B + (long long)k * g.ldb + n) : ld_c(B + (long long)n * g.ldb + k);
+        v.im *= b_conj;
+      }
+      rb[e] = v;
+    }
+  };
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int m = a_r + (a_kcontig ? 0 : e);
+      const int k = a_c + (a_kcontig ? e : 0);
+      As[k][m] = CR<R>{(R)ra[e].re, (R)ra[e].im};
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = b_r + (b_ncontig ? 0 : e);
+      const int n = b_c + (b_ncontig ? e : 0);
+      Bs[k][n] = CR<R>{(R)rb[e].re, (R)rb[e].im};
+    }
+  };
+
+  using V = typename Acc<R>::V;
+  V cre[2][2], cim[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      cre[i][j] = V{0, 0, 0, 0};
+      cim[i][j] = V{0, 0, 0, 0};
+    }
+
+  const int ktiles = (g.K + BK - 1) / BK;
+  load_tiles(0);
+  for (int kt = 0; kt < ktiles; ++kt) {
+    store_tiles();
+    __syncthreads();
+    if (kt + 1 < ktiles) load_tiles((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      CR<R> a[2], b[2];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        a[s] = As[kr][wm * 32 + s * 16 + (lane & 15)];
+        b[s] = Bs[kr][wn * 32 + s * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const R nai = -a[i].im;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          cre[i][j] = mfma(a[i].re, b[j].re, cre[i][j]);
+          cre[i][j] = mfma(nai, b[j].im, cre[i][j]);
+          cim[i][j] = mfma(a[i].re, b[j].im, cim[i][j]);
+          cim[i][j] = mfma(a[i].im, b[j].re, cim[i][j]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // epilogue
+  cplx* C = g.C + tm.z_in * g.sC_in + tm.z_out * g.sC_out;
+  const bool use_c0 = g.C0 != nullptr && (g.beta.re != 0.0 || g.beta.im != 0.0);
+  const cplx* C0 = use_c0 ? g.C0 + tm.z_in * g.sC0_in + tm.z_out * g.sC0_out : nullptr;
+  cplx sh{0.0, 0.0};
+  const bool use_shift = g.shift != nullptr;
+  if (use_shift) sh = g.shift[tm.z_out];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = tm.m0 + wm * 32 + i * 16 + acc_row<R>(lane, v);
+        const int col = tm.n0 + wn * 32 + j * 16 + (lane & 15);
+        if (row < g.M && col < g.N) {
+          const double xr = (double)cre[i][j][v], xi = (double)cim[i][j][v];
+          double outr = g.alpha.re * xr - g.alpha.im * xi;
+          double outi = g.alpha.re * xi + g.alpha.im * xr;
+          if (use_c0) {
+            const cplx c0 = ld_c(C0 + (long long)row * g.ldc0 + col);
+            outr += g.beta.re * c0.re - g.beta.im * c0.im;
+            outi += g.beta.re * c0.im + g.beta.im * c0.re;
+          }
+          if (use_shift && row == col) {
+            outr += sh.re;
+            outi += sh.im;
+          }
+          *reinterpret_cast<double2*>(C + (long long)row * g.ldc + col) = double2{outr, outi};
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise helpers (memory-bound; one thread per complex element)
+__global__ void axpby_shift_kernel(int M, int N, cplx a, const cplx* D, int ldd, long long sD_in,
+                                   long long sD_out, cplx* C, int ldc, long long sC_in,
+                                   long long sC_out, const cplx* shift, int inner) {
+  const int z = blockIdx.y;
+  const int zo = z / inner, zi = z - zo * inner;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)M * N) return;
+  const int r = (int)(e / N), c = (int)(e - (long long)r * N);
+  const cplx d = ld_c(D + zi * sD_in + zo * sD_out + (long long)r * ldd + c);
+  double outr = a.re * d.re - a.im * d.im;
+  double outi = a.re * d.im + a.im * d.re;
+  if (shift != nullptr && r == c) {
+    outr += shift[zo].re;
+    outi += shift[zo].im;
+  }
+  *reinterpret_cast<double2*>(C + zi * sC_in + zo * sC_out + (long long)r * ldc + c) =
+      double2{outr, outi};
+}
+
+// 32 x 32 tiles through LDS so both the read and the write are coalesced.
+__global__ __launch_bounds__(256) void transpose_kernel(int M, int N, double conj, const cplx* S,
+                                                        int lds, long long sS_in,
+                                                        long long sS_out, cplx* C, int ldc,
+                                                        long long sC_in, long long sC_out,
+                                                        int inner, int tiles_n) {
+  __shared__ cplx t[32][33];
+  const int z = blockIdx.y;
+  const int zo = z / inner, zi = z - zo * inner;
+  const int tile = blockIdx.x;
+  const int r0 = (tile / tiles_n) * 32, c0 = (tile % tiles_n) * 32;  // output coords
+  S += zi * sS_in + zo * sS_out;
+  C += zi * sC_in + zo * sC_out;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  // input is N x M: read S[c][r] for output (r, c)
+  for (int y = ty; y < 32; y += 8) {
+    const int sr = c0 + y, sc = r0 + tx;
+    if (sr < N && sc < M) t[y][tx] = ld_c(S + (long long)sr * lds + sc);
+  }
+  __syncthreads();
+  for (int y = ty; y < 32; y += 8) {
+    const int r = r0 + y, c = c0 + tx;
+    if (r < M && c < N) {
+      const cplx v = t[tx][y];
+      *reinterpret_cast<double2*>(C + (long long)r * ldc + c) = double2{v.re, conj * v.im};
+    }
+  }
+}
+
+__global__ void fill_kernel(cplx* p, long long n, cplx v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) *reinterpret_cast<double2*>(p + i) = double2{v.re, v.im};
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, long long n, unsigned long long v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+hipError_t launch_gemm(const GemmArgs& g, Prec p, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0 || g.inner <= 0 || g.outer <= 0) return hipSuccess;
+  if (g.K <= 0) {
+    // C = beta * C0 + shift: no product term
+    if (g.C0 == nullptr) return hipErrorInvalidValue;
+    return launch_axpby_shift(g.M, g.N, g.beta, g.C0, g.ldc0, g.sC0_in, g.sC0_out, g.C, g.ldc,
+                              g.sC_in, g.sC_out, g.shift, g.inner, g.outer, s);
+  }
+  const int tiles_m = (g.M + BM - 1) / BM;
+  const int tiles_n = (g.N + BN - 1) / BN;
+  const long long blocks = (long long)tiles_m * tiles_n * g.inner * g.outer;
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(gemm_kernel<double>, dim3((unsigned)blocks), dim3(NTHR), 0, s, g, tiles_m,
+                       tiles_n);
+  else
+    hipLaunchKernelGGL(gemm_kernel<float>, dim3((unsigned)blocks), dim3(NTHR), 0, s, g, tiles_m,
+                       tiles_n);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpby_shift(int M, int N, cplx a, const cplx* D, int ldd, long long sD_in,
+                              long long sD_out, cplx* C, int ldc, long long sC_in,
+                              long long sC_out, const cplx* shift, int inner, int outer,
+                              hipStream_t s) {
+  const long long n = (long long)M * N;
+  if (n <= 0 || inner * outer <= 0) return hipSuccess;
+  dim3 grid((unsigned)((n + 255) / 256), (unsigned)(inner * outer));
+  hipLaunchKernelGGL(axpby_shift_kernel, grid, dim3(256), 0, s, M, N, a, D, ldd, sD_in, sD_out, C,
+                     ldc, sC_in, sC_out, shift, inner);
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose(int M, int N, int op, const cplx* S, int lds, long long sS_in,
+                            long long sS_out, cplx* C, int ldc, long long sC_in,
+                            long long sC_out, int inner, int outer, hipStream_t s) {
+  if (M <= 0 || N <= 0 || inner * outer <= 0) return hipSuccess;
+  const int tm = (M + 31) / 32, tn = (N + 31) / 32;
+  dim3 grid((unsigned)(tm * tn), (unsigned)(inner * outer));
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, s, M, N, op == OP_C ? -1.0 : 1.0, S,
+                     lds, sS_in, sS_out, C, ldc, sC_in, sC_out, inner, tn);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(cplx* p, long long n, cplx v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, long long n, unsigned long long v,
+                           hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n,
+                     v);
+  return hipGetLastError();
+}
+
+}  // namespace sinv
+
+// ---------------------------------------------------------------------------
+// C ABI
+extern "C" int sinv_zgemm_batched(int M, int N, int K, int opA, int opB, sinv_c128 alpha,
+                                  const void* A, int lda, long long strideA, const void* B,
+                                  int ldb, long long strideB, sinv_c128 beta, const void* C0,
+                                  int ldc0, long long strideC0, void* C, int ldc,
+                                  long long strideC, const void* shift, int batch, int precision,
+                                  void* stream) {
+  using namespace sinv;
+  if (M < 0 || N < 0 || K < 0 || batch < 0 || opA < 0 || opA > 2 || opB < 0 || opB > 2 ||
+      (precision != SINV_PREC_FP64 && precision != SINV_PREC_MIXED) || C == nullptr ||
+      (K > 0 && (A == nullptr || B == nullptr))) {
+    set_error("sinv_zgemm_batched: invalid argument");
+    return SINV_E_INVAL;
+  }
+  const int arows = opA == OP_N ? M : K, acols = opA == OP_N ? K : M;
+  const int brows = opB == OP_N ? K : N, bcols = opB == OP_N ? N : K;
+  (void)arows;
+  (void)brows;
+  if ((K > 0 && (lda < acols || ldb < bcols)) || ldc < N || (C0 != nullptr && ldc0 < N)) {
+    set_error("sinv_zgemm_batched: leading dimension too small");
+    return SINV_E_INVAL;
+  }
+  if (K == 0 && C0 == nullptr) {
+    set_error("sinv_zgemm_batched: K == 0 needs C0");
+    return SINV_E_INVAL;
+  }
+  GemmArgs g{};
+  g.M = M; g.N = N; g.K = K; g.opA = opA; g.opB = opB;
+  g.A = (const cplx*)A; g.lda = lda; g.sA_in = strideA; g.sA_out = 0;
+  g.B = (const cplx*)B; g.ldb = ldb; g.sB_in = strideB; g.sB_out = 0;
+  g.C0 = (const cplx*)C0; g.ldc0 = ldc0; g.sC0_in = strideC0; g.sC0_out = 0;
+  g.C = (cplx*)C; g.ldc = ldc; g.sC_in = strideC; g.sC_out = 0;
+  g.alpha = cplx{alpha.re, alpha.im};
+  g.beta = cplx{beta.re, beta.im};
+  // shift is indexed by the outer batch index: make every entry its own outer index
+  g.shift = (const cplx*)shift;
+  if (shift != nullptr) {
+    g.inner = 1; g.outer = batch;
+    g.sA_out = strideA; g.sA_in = 0; g.sB_out = strideB; g.sB_in = 0;
+    g.sC0_out = strideC0; g.sC0_in = 0; g.sC_out = strideC; g.sC_in = 0;
+  } else {
+    g.inner = batch; g.outer = 1;
+  }
+  if (batch == 0) return SINV_OK;
+  SINV_HIP_TRY(launch_gemm(g, precision == SINV_PREC_FP64 ? Prec::F64 : Prec::F32,
+                           (hipStream_t)stream));
+  return SINV_OK;
+}

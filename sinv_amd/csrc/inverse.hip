@@ -1,0 +1,236 @@
+// K1: batched in-place complex128 block inverse.
+//
+// Leaves (n <= 64): one workgroup per matrix, the whole block resident in LDS
+// (64 x 65 complex128 = 65 KiB), Gauss-Jordan elimination with partial
+// pivoting.  The pivot is the first row maximising |re| + |im| in the column
+// (the choice LAPACK's izamax makes inside ?getrf, which numpy.linalg.inv
+// calls), found with one wave64 shuffle reduction per step.  Row swaps are
+// undone at the end as a column gather.
+//
+// Larger blocks: 2 x 2 Schur-complement recursion, all of whose arithmetic is
+// batched GEMM on FP64 MFMA:
+//   X11 = inv(A11); T21 = A21 X11; T12 = X11 A12; S = A22 - A21 T12
+//   Y = inv(S); A12 <- -T12 Y; A21 <- -Y T21; A11 <- X11 - A12 T21
+// Pivoting is therefore local to each leaf: the recursion needs nonsingular
+// leading sub-blocks, which diagonally dominant blocks and blocks whose
+// anti-Hermitian part is definite (the Schur complements of (E + i eta) I - H,
+// eta > 0) always have.  Cost 8 n^3 real flops either way.
+#include "sinv_core.hpp"
+
+namespace sinv {
+
+namespace {
+
+constexpr int LEAF = kLeaf;
+
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// Smith's complex reciprocal (no intermediate overflow for large moduli)
+__device__ __forceinline__ cplx crecip(cplx p) {
+  if (fabs(p.re) >= fabs(p.im)) {
+    const double r = p.im / p.re;
+    const double den = p.re + p.im * r;
+    return cplx{1.0 / den, -r / den};
+  } else {
+    const double r = p.re / p.im;
+    const double den = p.re * r + p.im;
+    return cplx{r / den, -1.0 / den};
+  }
+}
+
+__global__ __launch_bounds__(256) void inv_leaf_kernel(cplx* A, int n, int lda, long long s_in,
+                                                       long long s_out, int inner,
+                                                       unsigned long long* info, int tag) {
+  __shared__ cplx a[LEAF][LEAF + 1];
+  __shared__ cplx colk[LEAF];
+  __shared__ int piv[LEAF];
+  __shared__ int perm[LEAF];
+  __shared__ int sp;
+  __shared__ cplx spv;
+
+  const int z = blockIdx.x;
+  const int zo = z / inner, zi = z - zo * inner;
+  A += zi * s_in + zo * s_out;
+  const int tid = threadIdx.x;
+  const int j = tid & 63;   // column owned in the elimination step
+  const int i0 = tid >> 6;  // first row (rows i0, i0+4, ...)
+
+  for (int i = i0; i < n; i += 4)
+    if (j < n) {
+      const double2 v = *reinterpret_cast<const double2*>(A + (long long)i * lda + j);
+      a[i][j] = cplx{v.x, v.y};
+    }
+  __syncthreads();
+
+  bool singular = false;
+  for (int k = 0; k < n; ++k) {
+    // --- pivot search on wave 0: first argmax of |re| + |im| over rows k..n-1
+    if (tid < 64) {
+      double v = -1.0;
+      int idx = tid;
+      if (tid >= k && tid < n) v = fabs(a[tid][k].re) + fabs(a[tid][k].im);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off);
+        const int oi = __shfl_xor(idx, off);
+        if (ov > v || (ov == v && oi < idx)) {
+          v = ov;
+          idx = oi;
+        }
+      }
+      if (tid == 0) {
+        sp = idx;
+        piv[k] = idx;
+        spv = a[idx][k];
+      }
+      if (v == 0.0) singular = true;
+    }
+    __syncthreads();
+    const int p = sp;
+    const cplx d = crecip(spv);
+    // --- swap rows k and p, scale the pivot row, save the multipliers of column k
+    if (tid < n) {
+      const cplx tk = a[k][tid], tp = a[p][tid];
+      const cplx src = (tid == k) ? cplx{1.0, 0.0} : tp;
+      a[k][tid] = cmul(src, d);
+      if (p != k) a[p][tid] = tk;
+      if (tid != k && tid != p) colk[tid] = a[tid][k];
+      if (tid == k) colk[p] = (p != k) ? tk : cplx{0.0, 0.0};
+    }
+    __syncthreads();
+    // --- eliminate column k from every other row (rank-1 update)
+    if (j < n) {
+      const cplx rk = a[k][j];
+      for (int i = i0; i < n; i += 4) {
+        if (i == k) continue;
+        const cplx f = colk[i];
+        const cplx cur = (j == k) ? cplx{0.0, 0.0} : a[i][j];
+        a[i][j] = cplx{cur.re - (f.re * rk.re - f.im * rk.im), cur.im - (f.re * rk.im + f.im * rk.re)};
+      }
+    }
+    __syncthreads();
+  }
+
+  // inv(A) = inv(P A) P: replay the row swaps as column swaps in reverse order
+  if (tid == 0) {
+    for (int c = 0; c < n; ++c) perm[c] = c;
+    for (int k = n - 1; k >= 0; --k) {
+      const int t = perm[k];
+      perm[k] = perm[piv[k]];
+      perm[piv[k]] = t;
+    }
+  }
+  if (tid == 0 && singular && info != nullptr)
+    atomicMin(info, ((unsigned long long)(unsigned)tag << 32) | (unsigned long long)(unsigned)z);
+  __syncthreads();
+  for (int i = i0; i < n; i += 4)
+    if (j < n) {
+      const cplx v = a[i][perm[j]];
+      *reinterpret_cast<double2*>(A + (long long)i * lda + j) = double2{v.re, v.im};
+    }
+}
+
+int split_point(int n) {
+  int n1 = ((n / 2 + 15) / 16) * 16;
+  if (n1 >= n) n1 = n / 2;
+  return n1;
+}
+
+}  // namespace
+
+size_t inverse_workspace_elems(int n, int batch) {
+  if (n <= LEAF) return 0;
+  const int n1 = split_point(n), n2 = n - n1;
+  const size_t own = 2ull * (size_t)n1 * (size_t)n2 * (size_t)batch;
+  const size_t a = inverse_workspace_elems(n1, batch), b = inverse_workspace_elems(n2, batch);
+  return own + (a > b ? a : b);
+}
+
+hipError_t launch_inverse(cplx* A, int n, int lda, long long s_in, long long s_out, int inner,
+                          int outer, cplx* work, unsigned long long* info, int tag,
+                          hipStream_t s) {
+  if (n <= 0 || inner <= 0 || outer <= 0) return hipSuccess;
+  if (n <= LEAF) {
+    hipLaunchKernelGGL(inv_leaf_kernel, dim3((unsigned)(inner * outer)), dim3(256), 0, s, A, n,
+                       lda, s_in, s_out, inner, info, tag);
+    return hipGetLastError();
+  }
+  const int n1 = split_point(n), n2 = n - n1;
+  cplx* A11 = A;
+  cplx* A12 = A + n1;
+  cplx* A21 = A + (long long)n1 * lda;
+  cplx* A22 = A + (long long)n1 * lda + n1;
+  const long long w = (long long)n1 * n2;
+  const long long batch = (long long)inner * outer;
+  cplx* W21 = work;              // n2 x n1, ld n1
+  cplx* W12 = work + w * batch;  // n1 x n2, ld n2
+  cplx* sub = work + 2 * w * batch;
+
+  auto gemm = [&](int M, int N, int K, const cplx* Ap, int la, long long sa_in, long long sa_out,
+                  const cplx* Bp, int lb, long long sb_in, long long sb_out, cplx* Cp, int lc,
+                  long long sc_in, long long sc_out, cplx alpha, bool accumulate) {
+    GemmArgs g{};
+    g.M = M; g.N = N; g.K = K; g.opA = OP_N; g.opB = OP_N;
+    g.A = Ap; g.lda = la; g.sA_in = sa_in; g.sA_out = sa_out;
+    g.B = Bp; g.ldb = lb; g.sB_in = sb_in; g.sB_out = sb_out;
+    g.C = Cp; g.ldc = lc; g.sC_in = sc_in; g.sC_out = sc_out;
+    g.C0 = accumulate ? Cp : nullptr; g.ldc0 = lc; g.sC0_in = sc_in; g.sC0_out = sc_out;
+    g.alpha = alpha;
+    g.beta = accumulate ? cplx{1.0, 0.0} : cplx{0.0, 0.0};
+    g.shift = nullptr;
+    g.inner = inner; g.outer = outer;
+    return launch_gemm(g, Prec::F64, s);
+  };
+  const cplx one{1.0, 0.0}, mone{-1.0, 0.0};
+  const long long ws_in = w, ws_out = w * inner;
+
+  SINV_HIP_RET(launch_inverse(A11, n1, lda, s_in, s_out, inner, outer, sub, info, tag, s));
+  // T21 = A21 X11
+  SINV_HIP_RET(gemm(n2, n1, n1, A21, lda, s_in, s_out, A11, lda, s_in, s_out, W21, n1, ws_in,
+                    ws_out, one, false));
+  // T12 = X11 A12
+  SINV_HIP_RET(gemm(n1, n2, n1, A11, lda, s_in, s_out, A12, lda, s_in, s_out, W12, n2, ws_in,
+                    ws_out, one, false));
+  // S = A22 - A21 T12
+  SINV_HIP_RET(gemm(n2, n2, n1, A21, lda, s_in, s_out, W12, n2, ws_in, ws_out, A22, lda, s_in,
+                    s_out, mone, true));
+  SINV_HIP_RET(launch_inverse(A22, n2, lda, s_in, s_out, inner, outer, sub, info, tag, s));
+  // A12 = -T12 Y
+  SINV_HIP_RET(gemm(n1, n2, n2, W12, n2, ws_in, ws_out, A22, lda, s_in, s_out, A12, lda, s_in,
+                    s_out, mone, false));
+  // A21 = -Y T21
+  SINV_HIP_RET(gemm(n2, n1, n2, A22, lda, s_in, s_out, W21, n1, ws_in, ws_out, A21, lda, s_in,
+                    s_out, mone, false));
+  // A11 = X11 - A12 T21
+  SINV_HIP_RET(gemm(n1, n1, n2, A12, lda, s_in, s_out, W21, n1, ws_in, ws_out, A11, lda, s_in,
+                    s_out, mone, true));
+  return hipSuccess;
+}
+
+}  // namespace sinv
+
+extern "C" size_t sinv_zinv_workspace_bytes(int n, int batch) {
+  if (n <= 0 || batch <= 0) return 0;
+  return sinv::inverse_workspace_elems(n, batch) * sizeof(sinv::cplx);
+}
+
+extern "C" int sinv_zinv_batched(void* A, int n, int lda, long long strideA, int batch, void* work,
+                                 size_t work_bytes, unsigned long long* info, void* stream) {
+  using namespace sinv;
+  if (A == nullptr || n < 0 || batch < 0 || lda < n) {
+    set_error("sinv_zinv_batched: invalid argument");
+    return SINV_E_INVAL;
+  }
+  if (work_bytes < sinv_zinv_workspace_bytes(n, batch) || (n > kLeaf && work == nullptr)) {
+    set_error("sinv_zinv_batched: workspace too small (%zu < %zu)", work_bytes,
+              sinv_zinv_workspace_bytes(n, batch));
+    return SINV_E_INVAL;
+  }
+  if (n == 0 || batch == 0) return SINV_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (info != nullptr) SINV_HIP_TRY(launch_fill_u64(info, 1, SINV_INFO_OK, s));
+  SINV_HIP_TRY(launch_inverse((cplx*)A, n, lda, strideA, 0, batch, 1, (cplx*)work, info, 0, s));
+  return SINV_OK;
+}

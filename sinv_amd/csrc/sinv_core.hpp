@@ -1,0 +1,87 @@
+// Internal declarations shared by the HIP translation units of libsinv_hip.
+//
+// Storage convention: complex128 is two interleaved doubles (numpy / torch
+// layout), blocks are row-major with a leading dimension.  Every launcher here
+// is stream-ordered on the caller's hipStream_t and allocates nothing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/sinv.h"
+
+namespace sinv {
+
+struct cplx {
+  double re, im;
+};
+
+enum Op : int { OP_N = 0, OP_T = 1, OP_C = 2 };
+
+// C[z] = alpha * op(A[z]) * op(B[z]) + beta * C0[z] + shift[z] * I
+//   op(A) is M x K, op(B) is K x N, C is M x N.
+//   Batch index z = z_outer * inner + z_inner addresses
+//     X[z] = X + z_inner * sX_in + z_outer * sX_out   (element strides)
+//   so one launch covers (energies x block rows) without pointer arrays.
+//   shift (device, indexed by z_outer) may be null; C0 may alias C.
+struct GemmArgs {
+  int M, N, K;
+  int opA, opB;
+  const cplx* A;  int lda;  long long sA_in, sA_out;
+  const cplx* B;  int ldb;  long long sB_in, sB_out;
+  const cplx* C0; int ldc0; long long sC0_in, sC0_out;
+  cplx* C;        int ldc;  long long sC_in, sC_out;
+  cplx alpha, beta;
+  const cplx* shift;
+  int inner, outer;  // batch = inner * outer
+};
+
+enum class Prec : int { F64 = 0, F32 = 1 };
+
+// K2: batched complex GEMM on MFMA (FP64 16x16x4 for F64, FP32 16x16x4 for F32).
+hipError_t launch_gemm(const GemmArgs& g, Prec p, hipStream_t s);
+
+// C[z] = a * D[z] + shift[z_outer] * I  (elementwise, b x b blocks), batched as GemmArgs.
+hipError_t launch_axpby_shift(int M, int N, cplx a, const cplx* D, int ldd, long long sD_in,
+                              long long sD_out, cplx* C, int ldc, long long sC_in,
+                              long long sC_out, const cplx* shift, int inner, int outer,
+                              hipStream_t s);
+
+// C[z] = op(S[z])  (op = T or C), M x N output from N x M input.
+hipError_t launch_transpose(int M, int N, int op, const cplx* S, int lds, long long sS_in,
+                            long long sS_out, cplx* C, int ldc, long long sC_in,
+                            long long sC_out, int inner, int outer, hipStream_t s);
+
+// K1: in-place batched inverse of n x n blocks at A + z_in*sA_in + z_out*sA_out.
+// Leaves (n <= kLeaf) use Gauss-Jordan with partial pivoting in LDS; larger n
+// use 2x2 block recursion (Schur complement) with leaves at the bottom.
+// A singular leaf pivot records key = (tag << 32 | z) into info (atomic min).
+constexpr int kLeaf = 64;
+size_t inverse_workspace_elems(int n, int batch);
+hipError_t launch_inverse(cplx* A, int n, int lda, long long sA_in, long long sA_out, int inner,
+                          int outer, cplx* work, unsigned long long* info, int tag, hipStream_t s);
+
+// Fill n elements with a value (used for NaN off-diagonals and info reset).
+hipError_t launch_fill(cplx* p, long long n, cplx v, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, long long n, unsigned long long v, hipStream_t s);
+
+// thread-local last error text for sinv_last_error()
+void set_error(const char* fmt, ...);
+
+}  // namespace sinv
+
+#define SINV_HIP_TRY(expr)                                                        \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess) {                                                       \
+      ::sinv::set_error("%s:%d: %s", __FILE__, __LINE__, hipGetErrorString(_e)); \
+      return SINV_E_HIP;                                                          \
+    }                                                                             \
+  } while (0)
+
+#define SINV_HIP_RET(expr)                \
+  do {                                    \
+    hipError_t _e = (expr);               \
+    if (_e != hipSuccess) return _e;      \
+  } while (0)

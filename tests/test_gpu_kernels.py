@@ -1,0 +1,92 @@
+"""GPU parity of the two kernels under the hot path, through the C ABI.
+
+K2 (batched complex GEMM) against numpy's matmul, K1 (batched block inverse)
+against numpy.linalg.inv (the two library calls the reference makes,
+rgf.py:44-62, bcr_serial.py:121-383).
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand(rng, *shape):
+    return rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
+
+
+def _op(X, op):
+    return X if op == "N" else (np.swapaxes(X, -1, -2) if op == "T" else np.conj(np.swapaxes(X, -1, -2)))
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (16, 16, 4), (64, 64, 64), (65, 33, 17),
+                                   (128, 96, 200), (3, 130, 70)])
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("T", "N"), ("N", "C"), ("C", "T")])
+def test_zgemm_fp64(built_lib, M, N, K, opA, opB):
+    from sinv_amd import core
+    print(f"zgemm M={M} N={N} K={K} {opA}{opB}", flush=True)
+    rng = np.random.default_rng(M * 1000 + N * 10 + K)
+    batch = 3
+    A = _rand(rng, batch, *((M, K) if opA == "N" else (K, M)))
+    B = _rand(rng, batch, *((K, N) if opB == "N" else (N, K)))
+    C0 = _rand(rng, batch, M, N)
+    sh = _rand(rng, batch)
+    alpha, beta = 0.7 - 0.2j, -1.3 + 0.5j
+    C = core.zgemm_batched(A, B, opA=opA, opB=opB, alpha=alpha, beta=beta, C0=C0, shift=sh)
+    ref = alpha * (_op(A, opA) @ _op(B, opB)) + beta * C0
+    for z in range(batch):
+        ref[z][np.diag_indices(min(M, N))] += sh[z]
+    got = C.cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 1e-14 * max(K, 1) ** 0.5 * 10, err
+
+
+def test_zgemm_shared_operand_and_mixed(built_lib):
+    from sinv_amd import core
+    rng = np.random.default_rng(5)
+    A = _rand(rng, 100, 80)          # shared by the batch (stride 0)
+    B = _rand(rng, 4, 80, 120)
+    ref = A @ B
+    C = core.zgemm_batched(A, B).cpu().numpy()
+    assert np.abs(C - ref).max() / np.abs(ref).max() < 1e-14
+    Cm = core.zgemm_batched(A, B, precision="mixed").cpu().numpy()
+    em = np.abs(Cm - ref).max() / np.abs(ref).max()
+    assert 1e-9 < em < 1e-5, em   # complex64 operands, FP32 accumulation
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 17, 63, 64, 65, 100, 128, 200, 384, 416])
+def test_zinv_dominant(built_lib, n):
+    from sinv_amd import core
+    print(f"zinv dominant n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    batch = 4
+    A = _rand(rng, batch, n, n) + 2.0 * n * np.eye(n)
+    X = core.zinv_batched(A).cpu().numpy()
+    ref = np.linalg.inv(A)
+    err = np.linalg.norm(X - ref) / np.linalg.norm(ref)
+    assert err < 1e-14, err
+
+
+@pytest.mark.parametrize("n", [2, 5, 16, 33, 64])
+def test_zinv_needs_pivoting(built_lib, n):
+    """Leaves pivot like LAPACK: a random non-dominant block and one with a
+    zero leading entry invert to numpy's result."""
+    from sinv_amd import core
+    rng = np.random.default_rng(100 + n)
+    A = _rand(rng, 3, n, n)
+    A[1, 0, 0] = 0.0
+    A[2] = np.roll(np.eye(n), 1, axis=0) * (1 + 1j)  # permutation: every pivot off-diagonal
+    X = core.zinv_batched(A).cpu().numpy()
+    ref = np.linalg.inv(A)
+    for z in range(3):
+        err = np.linalg.norm(X[z] - ref[z]) / np.linalg.norm(ref[z])
+        assert err < 1e-15 * np.linalg.cond(A[z]) * n, (z, err)
+
+
+def test_zinv_singular_raises(built_lib):
+    from sinv_amd import core
+    A = np.zeros((2, 8, 8), complex)
+    A[0] = np.eye(8)
+    with pytest.raises(np.linalg.LinAlgError) as ei:
+        core.zinv_batched(A)
+    assert ei.value.instance == 1
