@@ -124,6 +124,19 @@ int sinv_zgemm_batched(int M, int N, int K, int opA, int opB, sinv_c128 alpha,
                        void* C, int ldc, long long strideC,
                        const void* shift, int batch, int precision, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Launch profiler (no counterpart in the reference, which has no timing code).
+ * Between sinv_prof_begin and sinv_prof_end every kernel launch of the GEMM
+ * and leaf-inverse kernels is bracketed by hipEvents on its own stream.
+ * sinv_prof_end synchronises those events and returns, per kind
+ * (0 = FP64 GEMM, 1 = FP32 GEMM, 2 = leaf inverse, 3 = other), the summed
+ * kernel time in ms, the launch count and the algorithmic flops
+ * (8 M N K per complex GEMM, 8 n^3 per inverse).  Process-wide state: use
+ * from one thread, around a region with at most `max_launches` launches.
+ * ------------------------------------------------------------------------ */
+int sinv_prof_begin(int max_launches);
+int sinv_prof_end(double* ms, long long* launches, double* flops, int nkinds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,6 +109,11 @@ def lib():
     L.sinv_zgemm_batched.restype = _i
     L.sinv_zgemm_batched.argtypes = [_i, _i, _i, _i, _i, C128, _vp, _i, _ll, _vp, _i, _ll,
                                      C128, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _i, _vp]
+    L.sinv_prof_begin.restype = _i
+    L.sinv_prof_begin.argtypes = [_i]
+    L.sinv_prof_end.restype = _i
+    L.sinv_prof_end.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_ll),
+                                ctypes.POINTER(ctypes.c_double), _i]
     if hasattr(L, "sinv_bcr_batched"):
         L.sinv_bcr_workspace_bytes.restype = ctypes.c_size_t
         L.sinv_bcr_workspace_bytes.argtypes = [ctypes.POINTER(BcrDesc)]
@@ -145,3 +150,29 @@ def check_info(info_tensor, what="inverse"):
         err = np.linalg.LinAlgError("Singular matrix")
         err.block, err.instance = block, inst
         raise err
+
+
+PROF_KINDS = ("gemm_f64", "gemm_f32", "inv_leaf", "other")
+
+
+class LaunchProfiler:
+    """Context manager around libsinv_hip's launch profiler.
+
+    ``stats`` maps kernel kind -> (total ms, launches, algorithmic flops)."""
+
+    def __init__(self, max_launches=200000):
+        self.max_launches = max_launches
+        self.stats = None
+
+    def __enter__(self):
+        check(lib().sinv_prof_begin(self.max_launches), "prof_begin")
+        return self
+
+    def __exit__(self, *exc):
+        n = len(PROF_KINDS)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_longlong * n)()
+        fl = (ctypes.c_double * n)()
+        check(lib().sinv_prof_end(ms, cnt, fl, n), "prof_end")
+        self.stats = {k: (ms[i], cnt[i], fl[i]) for i, k in enumerate(PROF_KINDS)}
+        return False

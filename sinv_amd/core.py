@@ -122,6 +122,55 @@ def rgf_batched(diag, upper, lower, *, sym_mat=False, save_off_diag=True, precis
     return Gd, Gu, Gl
 
 
+class RgfPlan:
+    """Pre-allocated batched RGF: inputs, outputs, workspace and descriptor are
+    bound once; ``run()`` only enqueues kernels on the current stream (nothing
+    is allocated or synchronised).  Same semantics as ``rgf_batched``."""
+
+    def __init__(self, diag, upper, lower, *, batch=None, sym_mat=False, save_off_diag=True,
+                 precision="fp64", shift=None, dscale=1.0, oscale=1.0, out=None):
+        if precision not in nat.PREC:
+            raise ValueError(f"precision must be one of {sorted(nat.PREC)}")
+        self.d, Bd, sd = _batched(to_dev(diag), 3)
+        N, b = self.d.shape[-3], self.d.shape[-1]
+        if N > 1:
+            self.u, Bu, su = _batched(to_dev(upper), 3)
+            self.l, Bl, sl = _batched(to_dev(lower), 3)
+        else:
+            self.u = self.l = None
+            Bu = Bl = None
+            su = sl = 0
+        self.shift = to_dev(shift).reshape(-1) if shift is not None else None
+        B = _resolve_batch(Bd, Bu, Bl, None if shift is None else self.shift.numel(), batch)
+        dev = device()
+        if out is None:
+            out = (torch.empty((B, N, b, b), dtype=_c128, device=dev),
+                   torch.empty((B, max(N - 1, 0), b, b), dtype=_c128, device=dev),
+                   torch.empty((B, max(N - 1, 0), b, b), dtype=_c128, device=dev))
+        self.Gd, self.Gu, self.Gl = out
+        self.info = torch.empty(1, dtype=torch.int64, device=dev)
+        self.desc = nat.RgfDesc(
+            nblocks=N, blocksize=b, bsizes=None, batch=B, sym_mat=int(bool(sym_mat)),
+            save_off_diag=int(bool(save_off_diag)), precision=nat.PREC[precision],
+            diag=_p(self.d), diag_stride=sd, upper=_p(self.u), upper_stride=su,
+            lower=_p(self.l), lower_stride=sl, dscale=nat.C128.of(dscale),
+            oscale=nat.C128.of(oscale), shift=_p(self.shift), g_diag=_p(self.Gd),
+            g_upper=_p(self.Gu), g_lower=_p(self.Gl), info=_info_ptr(self.info))
+        self.lib = nat.lib()
+        self.ws = _workspace(self.lib.sinv_rgf_workspace_bytes(ctypes.byref(self.desc)))
+        self.N, self.b, self.batch = N, b, B
+
+    def run(self):
+        nat.check(self.lib.sinv_rgf_batched(ctypes.byref(self.desc), _p(self.ws),
+                                             self.ws.numel(), _stream()), "rgf")
+
+    def check(self):
+        nat.check_info(self.info)
+
+    def outputs(self):
+        return self.Gd, self.Gu, self.Gl
+
+
 def rgf_energies(H_diag, H_upper, H_lower, energies, eta, **kw):
     """RGF of A(E_k) = (E_k + i*eta) I - H for every energy, H shared.
 

@@ -318,12 +318,15 @@ hipError_t launch_gemm(const GemmArgs& g, Prec p, hipStream_t s) {
   const int tiles_n = (g.N + BN - 1) / BN;
   const long long blocks = (long long)tiles_m * tiles_n * g.inner * g.outer;
   if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+  const int pid = prof_open(s);
   if (p == Prec::F64)
     hipLaunchKernelGGL(gemm_kernel<double>, dim3((unsigned)blocks), dim3(NTHR), 0, s, g, tiles_m,
                        tiles_n);
   else
     hipLaunchKernelGGL(gemm_kernel<float>, dim3((unsigned)blocks), dim3(NTHR), 0, s, g, tiles_m,
                        tiles_n);
+  prof_close(pid, s, p == Prec::F64 ? PROF_GEMM_F64 : PROF_GEMM_F32,
+             8.0 * g.M * (double)g.N * g.K * g.inner * (double)g.outer);
   return hipGetLastError();
 }
 

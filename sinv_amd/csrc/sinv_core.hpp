@@ -69,6 +69,12 @@ hipError_t launch_fill_u64(unsigned long long* p, long long n, unsigned long lon
 // thread-local last error text for sinv_last_error()
 void set_error(const char* fmt, ...);
 
+// launch profiler (prof.hip): prof_open before a launch returns an id (-1 when
+// off), prof_close after it records the kernel kind and its algorithmic flops.
+enum ProfKind : int { PROF_GEMM_F64 = 0, PROF_GEMM_F32 = 1, PROF_INV_LEAF = 2, PROF_OTHER = 3 };
+int prof_open(hipStream_t s);
+void prof_close(int id, hipStream_t s, int kind, double flops);
+
 }  // namespace sinv
 
 #define SINV_HIP_TRY(expr)                                                        \

@@ -1,0 +1,80 @@
+"""Host-side logic of the drop-in layer (no GPU): the bsparse-like container,
+the converters and generators of matrices_utils.py, and the workload
+generators."""
+
+import numpy as np
+import pytest
+
+from oracle import matu_ref as mu
+from sinv_amd.bsparse import BlockTridiag
+from sinv_amd.utils import configs, matu
+
+
+def test_generators_match_reference_streams():
+    for cplx in (False, True):
+        for sym in (False, True):
+            a = matu.generateRandomNumpyMat(17, cplx, sym, 63)
+            b = mu.generateRandomNumpyMat(17, cplx, sym, 63)
+            assert np.array_equal(a, b)
+            a = matu.generateBandedDiagonalMatrix(40, 3, cplx, sym, 63)
+            b = mu.generateBandedDiagonalMatrix(40, 3, cplx, sym, 63)
+            assert np.array_equal(a, b)
+
+
+def test_banded_mask_against_loop():
+    """matrices_utils.py:86-89 loops over all entries; the vectorised mask agrees."""
+    A = matu.generateBandedDiagonalMatrix(23, 4.0, True, False, 5)
+    B = mu.generateRandomNumpyMat(23, True, False, 5)
+    for i in range(23):
+        for j in range(23):
+            if i - j > 4 or j - i > 4:
+                B[i, j] = 0
+    assert np.array_equal(A, B)
+
+
+def test_converters_round_trip():
+    A = matu.generateBandedDiagonalMatrix(48, 6, True, False, 1)
+    d, u, l = matu.convertDenseToBlkTridiag(A, 8)
+    assert d.shape == (6, 8, 8) and u.shape == (5, 8, 8)
+    assert np.array_equal(matu.convertBlkTridiagToDense(d, u, l), A)
+    d2, u2, l2 = mu.convertDenseToBlkTridiag(A, 8)
+    assert np.array_equal(d, d2) and np.array_equal(u, u2) and np.array_equal(l, l2)
+
+
+def test_container_surface():
+    d, u, l = configs.dominant_blocks(5, 3, seed=0)
+    A = BlockTridiag(d, u, l)
+    assert A.blockorder == 5 and A.blocksize == 3 and A.shape == (15, 15)
+    assert np.array_equal(A.blocks[2, 3], u[2]) and np.array_equal(A.blocks[3, 2], l[2])
+    assert not A.blocks[0, 4].any()
+    G = A.copy() * np.nan
+    assert np.isnan(G.diag).all() and not np.isnan(A.diag).any()
+    G.blocks[1, 1] = d[1]
+    assert np.array_equal(G.diag[1], d[1])
+    with pytest.raises(IndexError):
+        G.blocks[0, 2] = d[0]
+    assert np.array_equal(A.toarray(), matu.convertBlkTridiagToDense(d, u, l))
+    B = BlockTridiag.from_dense(A.toarray(), 3)
+    assert np.array_equal(B.toarray(), A.toarray())
+
+
+def test_workload_generators():
+    d, u, l = configs.dominant_blocks(4, 8, seed=1)
+    A = matu.convertBlkTridiagToDense(d, u, l)
+    assert np.array_equal(A, A.T)                      # complex symmetric
+    off = np.abs(A).sum(1) - np.abs(np.diag(A))
+    assert (np.abs(np.diag(A)) > off).all()             # strictly diagonally dominant
+    H = configs.hermitian_H(4, 8, seed=1)
+    Hd = matu.convertBlkTridiagToDense(*H)
+    assert np.allclose(Hd, Hd.conj().T)
+    assert configs.rgf_flops(16, 64) == 8 * 64 ** 3 * (9 * 16 - 8)
+    assert configs.rgf_flops_ragged([64] * 16) == configs.rgf_flops(16, 64)
+
+
+def test_product_path_has_no_oracle_import():
+    """The product package never imports the oracle (it is test infrastructure)."""
+    import pathlib
+    root = pathlib.Path(__file__).resolve().parents[1] / "sinv_amd"
+    for f in root.rglob("*.py"):
+        txt = f.read_text()
+        assert "import oracle" not in txt and "from oracle" not in txt, f
