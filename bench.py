@@ -120,13 +120,15 @@ def main():
 
     stream = torch.cuda.current_stream()
 
-    def timed(plans, steps, warmup):
+    def timed(plans, steps, warmup, tag):
         for _ in range(warmup):
             for p in plans:
                 p.run()
         torch.cuda.synchronize()
         for p in plans:
             p.check()
+        # roctx range: lets a rocprofv3 --marker-trace run select this region's dispatches
+        torch.cuda.nvtx.range_push(f"sinv_bench_timed:{tag}")
         t0 = time.perf_counter()
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
@@ -137,13 +139,14 @@ def main():
         ev1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        torch.cuda.nvtx.range_pop()
         return ev0.elapsed_time(ev1) / steps, wall / steps
 
     results = {}
     for prec in ("fp64", "mixed"):
         log(f"[bench] {wl['workload']}: batched {prec}: allocating plans")
         plans = make_plans(prec)
-        ms, wall = timed(plans, args.steps, args.warmup)
+        ms, wall = timed(plans, args.steps, args.warmup, prec)
         log(f"[bench] batched {prec}: {ms:.1f} ms/step (wall {wall * 1e3:.1f}), "
             f"{solves / (ms / 1e3):.1f} solves/s, {alg_flops / (ms / 1e3) / 1e12:.2f} TFLOP/s")
         with _native.LaunchProfiler() as prof:

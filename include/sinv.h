@@ -98,6 +98,41 @@ size_t sinv_rgf_workspace_bytes(const sinv_rgf_desc* d);
 int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
+ * BCR-serial selected inversion, batched over instances.
+ * Replaces bcr_s.bcr_serial (src/sinv/algorithms/bcr/bcr_serial.py:19-62):
+ * reduce_bcr/reduce (:173-216, :66-169), invert_block (:220-249),
+ * produce_bcr with corner/center production (:387-466, :253-383).  The
+ * identity padding of bcr_utils.py:35-71 is skipped (padded rows contribute
+ * exact zeros).  Instance blocks as in sinv_rgf_desc (uniform block size,
+ * A_ii = dscale*diag + shift*I, couplings = oscale*upper/lower).
+ * Outputs: diagonal / upper / lower blocks of G; optionally (g_fill != NULL)
+ * the other blocks the reference's dense result holds -- G between the
+ * active rows r_m = m*2^l + 2^l - 1 of each level l >= 1, per instance
+ * level by level: G[r_c, r_{c+1}] for c < n_l - 1, then G[r_{c+1}, r_c];
+ * sinv_bcr_fill_blocks(N) blocks per instance.
+ * ------------------------------------------------------------------------ */
+typedef struct sinv_bcr_desc {
+  int nblocks;
+  int blocksize;
+  int batch;
+  const void* diag;  long long diag_stride;
+  const void* upper; long long upper_stride;
+  const void* lower; long long lower_stride;
+  sinv_c128 dscale;
+  sinv_c128 oscale;
+  const void* shift;
+  void* g_diag;
+  void* g_upper;
+  void* g_lower;
+  void* g_fill;
+  unsigned long long* info;
+} sinv_bcr_desc;
+
+size_t sinv_bcr_workspace_bytes(const sinv_bcr_desc* d);
+long long sinv_bcr_fill_blocks(int nblocks);
+int sinv_bcr_batched(const sinv_bcr_desc* d, void* work, size_t work_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
  * Batched in-place complex128 block inverse (kernel K1).
  * Replaces the numpy.linalg.inv calls of the hot path (rgf.py:44,48;
  * bcr_serial.py:121,148,249,302,379).  Blocks with n <= 64 use Gauss-Jordan

@@ -117,6 +117,8 @@ def lib():
     if hasattr(L, "sinv_bcr_batched"):
         L.sinv_bcr_workspace_bytes.restype = ctypes.c_size_t
         L.sinv_bcr_workspace_bytes.argtypes = [ctypes.POINTER(BcrDesc)]
+        L.sinv_bcr_fill_blocks.restype = _ll
+        L.sinv_bcr_fill_blocks.argtypes = [_i]
         L.sinv_bcr_batched.restype = _i
         L.sinv_bcr_batched.argtypes = [ctypes.POINTER(BcrDesc), _vp, ctypes.c_size_t, _vp]
     if hasattr(L, "sinv_rgf_adjoint_batched"):

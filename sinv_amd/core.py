@@ -171,6 +171,66 @@ class RgfPlan:
         return self.Gd, self.Gu, self.Gl
 
 
+def bcr_batched(diag, upper, lower, *, shift=None, dscale=1.0, oscale=1.0, return_fill=False,
+                check=True):
+    """BCR-serial selected inversion of a batch of instances (bcr_serial.py:19-466).
+
+    Same instance convention as ``rgf_batched`` (uniform block size).  Returns
+    (G_diag, G_upper, G_lower) and, with ``return_fill``, the produced blocks
+    of levels >= 1 as (B, F, b, b) in the order documented in include/sinv.h.
+    """
+    d, Bd, sd = _batched(to_dev(diag), 3)
+    N, b = d.shape[-3], d.shape[-1]
+    if N > 1:
+        u, Bu, su = _batched(to_dev(upper), 3)
+        l, Bl, sl = _batched(to_dev(lower), 3)
+        if u.shape[-3:] != (N - 1, b, b) or l.shape[-3:] != (N - 1, b, b):
+            raise ValueError("upper/lower must hold N-1 blocks of the diagonal block size")
+    else:
+        u = l = None
+        Bu = Bl = None
+        su = sl = 0
+    sh = to_dev(shift).reshape(-1) if shift is not None else None
+    B = _resolve_batch(Bd, Bu, Bl, None if sh is None else sh.numel())
+    dev = device()
+    L = nat.lib()
+    Gd = torch.empty((B, N, b, b), dtype=_c128, device=dev)
+    Gu = torch.empty((B, max(N - 1, 0), b, b), dtype=_c128, device=dev)
+    Gl = torch.empty((B, max(N - 1, 0), b, b), dtype=_c128, device=dev)
+    F = int(L.sinv_bcr_fill_blocks(N))
+    Gf = torch.empty((B, F, b, b), dtype=_c128, device=dev) if (return_fill and F > 0) else None
+    info = torch.empty(1, dtype=torch.int64, device=dev)
+    desc = nat.BcrDesc(
+        nblocks=N, blocksize=b, batch=B, diag=_p(d), diag_stride=sd, upper=_p(u), upper_stride=su,
+        lower=_p(l), lower_stride=sl, dscale=nat.C128.of(dscale), oscale=nat.C128.of(oscale),
+        shift=_p(sh), g_diag=_p(Gd), g_upper=_p(Gu), g_lower=_p(Gl), g_fill=_p(Gf),
+        info=_info_ptr(info))
+    ws = _workspace(L.sinv_bcr_workspace_bytes(ctypes.byref(desc)))
+    nat.check(L.sinv_bcr_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()), "bcr")
+    if check:
+        nat.check_info(info)
+    if return_fill:
+        if Gf is None:
+            Gf = torch.empty((B, 0, b, b), dtype=_c128, device=dev)
+        return Gd, Gu, Gl, Gf
+    return Gd, Gu, Gl
+
+
+def bcr_fill_positions(N):
+    """(row, col) block positions of the blocks bcr_batched(return_fill=True) returns."""
+    out = []
+    s = 2
+    while True:
+        n = N // s
+        if n < 1:
+            break
+        rows = [m * s + s - 1 for m in range(n)]
+        out += [(rows[c], rows[c + 1]) for c in range(n - 1)]
+        out += [(rows[c + 1], rows[c]) for c in range(n - 1)]
+        s *= 2
+    return out
+
+
 def rgf_energies(H_diag, H_upper, H_lower, energies, eta, **kw):
     """RGF of A(E_k) = (E_k + i*eta) I - H for every energy, H shared.
 

@@ -1,0 +1,84 @@
+"""GPU parity of BCR-serial (bcr_serial.py:19-466) against the oracle.
+
+The oracle's dense restatement (oracle/bcr_ref.py) is pinned to the dense
+inverse in tests/test_oracle_bcr.py; its compressed schedule is bitwise the
+dense one.  Inputs are the reference test's generator and shapes (SEED = 63,
+bcr_serial_test.py:40-65).
+"""
+
+import numpy as np
+import pytest
+
+from oracle import bcr_ref, dense
+from oracle import matu_ref as mu
+
+pytestmark = pytest.mark.gpu
+
+SEED = 63
+BCR_SHAPES = [(9, 1), (18, 2), (27, 3), (12, 1), (24, 2), (36, 3), (256, 8), (240, 10), (144, 12)]
+
+
+@pytest.mark.parametrize("n,b", BCR_SHAPES)
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("sym", [False, True])
+def test_bcr_serial_dense_dropin(built_lib, n, b, cplx, sym):
+    from sinv_amd.algorithms import bcr_s
+    print(f"bcr ref-shape n={n} b={b} cplx={cplx} sym={sym}", flush=True)
+    A = mu.generateBandedDiagonalMatrix(n, np.ceil(b / 2), cplx, sym, SEED)
+    G = bcr_s.bcr_serial(A, b)
+    assert G.dtype == A.dtype and G.shape == A.shape
+    # the reference's acceptance test (bcr_serial_test.py:66-73)
+    got = mu.convertDenseToBlkTridiag(G, b)
+    ref = mu.convertDenseToBlkTridiag(np.linalg.inv(A), b)
+    for g, r in zip(got, ref):
+        assert np.allclose(g, r)
+    # every block of the reference's dense output, fill-in included
+    Gref = bcr_ref.bcr_serial_dense(A, b)
+    cond = np.linalg.cond(A)
+    err = np.linalg.norm(G - Gref) / np.linalg.norm(Gref)
+    assert err < 1e-15 * cond * 10, err
+    # nothing outside the blocks the reduction produces
+    from sinv_amd import core
+    N = n // b
+    mask = np.zeros(G.shape, bool)
+    pos = [(i, i) for i in range(N)] + [(i, i + 1) for i in range(N - 1)] + \
+        [(i + 1, i) for i in range(N - 1)] + core.bcr_fill_positions(N)
+    for r, c in pos:
+        mask[r * b:(r + 1) * b, c * b:(c + 1) * b] = True
+    assert not np.any(G[~mask])
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 7, 8, 15, 16, 17, 33])
+def test_bcr_block_counts(built_lib, N):
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    d, u, l = configs.dominant_blocks(N, 24, seed=N)
+    Gd, Gu, Gl = core.bcr_batched(d, u, l)
+    got = tuple(x[0].cpu().numpy() for x in (Gd, Gu, Gl))
+    ref = bcr_ref.bcr_blocks(d, u, l)
+    assert dense.rel_err(got, ref) < 1e-13
+
+
+def test_bcr_dominant_16x64(built_lib):
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    d, u, l = configs.dominant_blocks(16, 64, seed=1)
+    got = tuple(x[0].cpu().numpy() for x in core.bcr_batched(d, u, l))
+    assert dense.rel_err(got, bcr_ref.bcr_blocks(d, u, l)) < 1e-12
+    assert dense.rel_err(got, dense.selected_inverse_dense(d, u, l)) < 1e-12
+
+
+def test_bcr_energies_agrees_with_rgf(built_lib):
+    """Batched over energies with A(E) formed on the fly; BCR and RGF agree."""
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    H = configs.hermitian_H(13, 72, seed=2)
+    E = configs.energies(4, seed=3) + 1e-2j
+    Gb = core.bcr_batched(*H, shift=E, dscale=-1.0, oscale=-1.0)
+    Gr = core.rgf_batched(*H, shift=E, dscale=-1.0, oscale=-1.0)
+    for k in range(len(E)):
+        b_ = tuple(x[k].cpu().numpy() for x in Gb)
+        r_ = tuple(x[k].cpu().numpy() for x in Gr)
+        assert dense.rel_err(b_, r_) < 1e-11
+        d, u, l = configs.shifted(H, E[k])
+        assert dense.residuals(d, u, l, *b_) < 1e-11
