@@ -98,6 +98,43 @@ size_t sinv_rgf_workspace_bytes(const sinv_rgf_desc* d);
 int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Adjoint of the RGF selected inversion, batched over instances.
+ * No counterpart in the reference (it has no gradients); it differentiates
+ * the map computed by rgf.py:40-62.  Given grad_G on the block-tridiagonal
+ * pattern (torch's convention for a real loss: dL/dRe + i dL/dIm), returns
+ * grad_A = -G^H grad_G G^H restricted to the pattern, for the same instance
+ * convention as sinv_rgf_desc (uniform block size).  For
+ * A(E) = (E + i eta) I - H the gradient of H is -grad_A.
+ * checkpoint_every = k > 0 stores the left-connected blocks only every k
+ * blocks and recomputes each k-block segment during the backward sweep
+ * (0 = store all).  gbar_* strides may be 0 (one grad_G for every instance).
+ * ------------------------------------------------------------------------ */
+typedef struct sinv_rgf_adjoint_desc {
+  int nblocks;
+  int blocksize;
+  int batch;
+  int precision;
+  int checkpoint_every;
+  const void* diag;  long long diag_stride;
+  const void* upper; long long upper_stride;
+  const void* lower; long long lower_stride;
+  sinv_c128 dscale;
+  sinv_c128 oscale;
+  const void* shift;
+  const void* gbar_diag;  long long gbar_diag_stride;
+  const void* gbar_upper; long long gbar_upper_stride;
+  const void* gbar_lower; long long gbar_lower_stride;
+  void* abar_diag;
+  void* abar_upper;
+  void* abar_lower;
+  unsigned long long* info;
+} sinv_rgf_adjoint_desc;
+
+size_t sinv_rgf_adjoint_workspace_bytes(const sinv_rgf_adjoint_desc* d);
+int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* work, size_t work_bytes,
+                             void* stream);
+
+/* ------------------------------------------------------------------------
  * BCR-serial selected inversion, batched over instances.
  * Replaces bcr_s.bcr_serial (src/sinv/algorithms/bcr/bcr_serial.py:19-62):
  * reduce_bcr/reduce (:173-216, :66-169), invert_block (:220-249),

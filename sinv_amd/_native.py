@@ -75,9 +75,7 @@ class AdjDesc(ctypes.Structure):
         ("gbar_diag", _vp), ("gbar_diag_stride", _ll),
         ("gbar_upper", _vp), ("gbar_upper_stride", _ll),
         ("gbar_lower", _vp), ("gbar_lower_stride", _ll),
-        ("g_diag", _vp), ("g_upper", _vp), ("g_lower", _vp),
         ("abar_diag", _vp), ("abar_upper", _vp), ("abar_lower", _vp),
-        ("accumulate", _i),
         ("info", ctypes.POINTER(ctypes.c_ulonglong)),
     ]
 

@@ -216,6 +216,56 @@ def bcr_batched(diag, upper, lower, *, shift=None, dscale=1.0, oscale=1.0, retur
     return Gd, Gu, Gl
 
 
+def rgf_adjoint_batched(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *, shift=None,
+                        dscale=1.0, oscale=1.0, precision="fp64", checkpoint_every=0,
+                        check=True):
+    """Gradient of a real loss w.r.t. the blocks of A, given its gradient w.r.t.
+    the selected blocks of G = A^-1: grad_A = -G^H grad_G G^H on the pattern.
+
+    Instance convention as ``rgf_batched``; the grad_G blocks may be shared by
+    all instances (no batch dimension).  Returns (grad_diag, grad_upper,
+    grad_lower), each with a batch dimension.  ``checkpoint_every = k`` keeps
+    the forward's left-connected blocks every k blocks only and recomputes the
+    rest segment by segment.
+    """
+    if precision not in nat.PREC:
+        raise ValueError(f"precision must be one of {sorted(nat.PREC)}")
+    d, Bd, sd = _batched(to_dev(diag), 3)
+    N, b = d.shape[-3], d.shape[-1]
+    gd, Bgd, sgd = _batched(to_dev(gbar_diag), 3)
+    if N > 1:
+        u, Bu, su = _batched(to_dev(upper), 3)
+        l, Bl, sl = _batched(to_dev(lower), 3)
+        gu, Bgu, sgu = _batched(to_dev(gbar_upper), 3)
+        gl, Bgl, sgl = _batched(to_dev(gbar_lower), 3)
+    else:
+        u = l = gu = gl = None
+        Bu = Bl = Bgu = Bgl = None
+        su = sl = sgu = sgl = 0
+    sh = to_dev(shift).reshape(-1) if shift is not None else None
+    B = _resolve_batch(Bd, Bu, Bl, Bgd, Bgu, Bgl, None if sh is None else sh.numel())
+    dev = device()
+    Ad = torch.empty((B, N, b, b), dtype=_c128, device=dev)
+    Au = torch.empty((B, max(N - 1, 0), b, b), dtype=_c128, device=dev)
+    Al = torch.empty((B, max(N - 1, 0), b, b), dtype=_c128, device=dev)
+    info = torch.empty(1, dtype=torch.int64, device=dev)
+    desc = nat.AdjDesc(
+        nblocks=N, blocksize=b, batch=B, precision=nat.PREC[precision],
+        checkpoint_every=int(checkpoint_every),
+        diag=_p(d), diag_stride=sd, upper=_p(u), upper_stride=su, lower=_p(l), lower_stride=sl,
+        dscale=nat.C128.of(dscale), oscale=nat.C128.of(oscale), shift=_p(sh),
+        gbar_diag=_p(gd), gbar_diag_stride=sgd, gbar_upper=_p(gu), gbar_upper_stride=sgu,
+        gbar_lower=_p(gl), gbar_lower_stride=sgl,
+        abar_diag=_p(Ad), abar_upper=_p(Au), abar_lower=_p(Al), info=_info_ptr(info))
+    L = nat.lib()
+    ws = _workspace(L.sinv_rgf_adjoint_workspace_bytes(ctypes.byref(desc)))
+    nat.check(L.sinv_rgf_adjoint_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()),
+              "rgf_adjoint")
+    if check:
+        nat.check_info(info)
+    return Ad, Au, Al
+
+
 def bcr_fill_positions(N):
     """(row, col) block positions of the blocks bcr_batched(return_fill=True) returns."""
     out = []

@@ -265,11 +265,14 @@ __global__ void axpby_shift_kernel(int M, int N, cplx a, const cplx* D, int ldd,
       double2{outr, outi};
 }
 
-// 32 x 32 tiles through LDS so both the read and the write are coalesced.
-__global__ __launch_bounds__(256) void transpose_kernel(int M, int N, double conj, const cplx* S,
-                                                        int lds, long long sS_in,
+// C = alpha op(S) + shift I, op = T or C.  32 x 32 tiles through LDS so both
+// the read and the write are coalesced.  shift (indexed by the outer batch
+// index) is conjugated when conj_shift is set.
+__global__ __launch_bounds__(256) void transpose_kernel(int M, int N, double conj, cplx alpha,
+                                                        const cplx* S, int lds, long long sS_in,
                                                         long long sS_out, cplx* C, int ldc,
                                                         long long sC_in, long long sC_out,
+                                                        const cplx* shift, int conj_shift,
                                                         int inner, int tiles_n) {
   __shared__ cplx t[32][33];
   const int z = blockIdx.y;
@@ -278,6 +281,11 @@ __global__ __launch_bounds__(256) void transpose_kernel(int M, int N, double con
   const int r0 = (tile / tiles_n) * 32, c0 = (tile % tiles_n) * 32;  // output coords
   S += zi * sS_in + zo * sS_out;
   C += zi * sC_in + zo * sC_out;
+  cplx sh{0.0, 0.0};
+  if (shift != nullptr) {
+    sh = shift[zo];
+    if (conj_shift) sh.im = -sh.im;
+  }
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   // input is N x M: read S[c][r] for output (r, c)
   for (int y = ty; y < 32; y += 8) {
@@ -289,7 +297,14 @@ __global__ __launch_bounds__(256) void transpose_kernel(int M, int N, double con
     const int r = r0 + y, c = c0 + tx;
     if (r < M && c < N) {
       const cplx v = t[tx][y];
-      *reinterpret_cast<double2*>(C + (long long)r * ldc + c) = double2{v.re, conj * v.im};
+      const double vr = v.re, vi = conj * v.im;
+      double outr = alpha.re * vr - alpha.im * vi;
+      double outi = alpha.re * vi + alpha.im * vr;
+      if (r == c) {
+        outr += sh.re;
+        outi += sh.im;
+      }
+      *reinterpret_cast<double2*>(C + (long long)r * ldc + c) = double2{outr, outi};
     }
   }
 }
@@ -344,12 +359,14 @@ hipError_t launch_axpby_shift(int M, int N, cplx a, const cplx* D, int ldd, long
 
 hipError_t launch_transpose(int M, int N, int op, const cplx* S, int lds, long long sS_in,
                             long long sS_out, cplx* C, int ldc, long long sC_in,
-                            long long sC_out, int inner, int outer, hipStream_t s) {
+                            long long sC_out, int inner, int outer, hipStream_t s, cplx alpha,
+                            const cplx* shift, bool conj_shift) {
   if (M <= 0 || N <= 0 || inner * outer <= 0) return hipSuccess;
   const int tm = (M + 31) / 32, tn = (N + 31) / 32;
   dim3 grid((unsigned)(tm * tn), (unsigned)(inner * outer));
-  hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, s, M, N, op == OP_C ? -1.0 : 1.0, S,
-                     lds, sS_in, sS_out, C, ldc, sC_in, sC_out, inner, tn);
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, s, M, N, op == OP_C ? -1.0 : 1.0,
+                     alpha, S, lds, sS_in, sS_out, C, ldc, sC_in, sC_out, shift,
+                     conj_shift ? 1 : 0, inner, tn);
   return hipGetLastError();
 }
 

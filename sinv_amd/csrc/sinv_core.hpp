@@ -48,10 +48,13 @@ hipError_t launch_axpby_shift(int M, int N, cplx a, const cplx* D, int ldd, long
                               long long sC_out, const cplx* shift, int inner, int outer,
                               hipStream_t s);
 
-// C[z] = op(S[z])  (op = T or C), M x N output from N x M input.
+// C[z] = alpha op(S[z]) + shift[z_outer] I  (op = T or C), M x N output from an
+// N x M input; the shift is conjugated when conj_shift is set.
 hipError_t launch_transpose(int M, int N, int op, const cplx* S, int lds, long long sS_in,
                             long long sS_out, cplx* C, int ldc, long long sC_in,
-                            long long sC_out, int inner, int outer, hipStream_t s);
+                            long long sC_out, int inner, int outer, hipStream_t s,
+                            cplx alpha = cplx{1.0, 0.0}, const cplx* shift = nullptr,
+                            bool conj_shift = false);
 
 // K1: in-place batched inverse of n x n blocks at A + z_in*sA_in + z_out*sA_out.
 // Leaves (n <= kLeaf) use Gauss-Jordan with partial pivoting in LDS; larger n
