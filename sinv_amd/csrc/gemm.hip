@@ -5,12 +5,13 @@
 // Design (gfx950, wave64):
 //  * workgroup = 256 threads = 4 waves in a 2 x 2 arrangement; workgroup tile
 //    64 x 64 complex, wave tile 32 x 32 = 2 x 2 MFMA tiles of 16 x 16;
-//  * K advances in chunks of 16 complex staged through LDS, with the next
-//    chunk's global loads issued into registers before the current chunk's
-//    MFMAs (single LDS buffer, register prefetch);
-//  * a complex product is 4 real MFMAs (re*re, -im*im, re*im, im*re) on
-//    v_mfma_f64_16x16x4_f64 (FP64) or v_mfma_f32_16x16x4_f32 (complex64 mode,
-//    operands rounded to float when staged into LDS, FP32 accumulation);
+//  * K advances in chunks of 16 complex staged through two LDS buffers: the
+//    next chunk's global loads are issued before the current chunk's MFMAs and
+//    land in the other buffer afterwards (one barrier per chunk);
+//  * a complex product is 3 real MFMAs on v_mfma_f64_16x16x4_f64 (3M:
+//    re*re, im*im, (re+im)*(re+im)) in FP64, and 4 on v_mfma_f32_16x16x4_f32
+//    in complex64 mode (operands rounded to float when staged into LDS, FP32
+//    accumulation);
 //  * op(A) and op(B) tiles are written to LDS k-major, so each MFMA operand
 //    read is 16 consecutive complex per quarter-wave;
 //  * the linear workgroup id is remapped so that consecutive tiles (same GEMM,
@@ -67,6 +68,11 @@ __device__ __forceinline__ int acc_row<float>(int lane, int v) {
   return (lane >> 4) * 4 + v;
 }
 
+// complex product form: 3 real MFMAs (3M) for FP64, 4 for FP32 (3M's extra
+// cancellation is not worth it at single precision)
+template <typename R>
+constexpr bool kThreeM = sizeof(R) == 8;
+
 constexpr int BM = 64;
 constexpr int BN = 64;
 constexpr int BK = 16;
@@ -103,9 +109,10 @@ __device__ __forceinline__ cplx ld_c(const cplx* p) {
 }
 
 template <typename R>
-__global__ __launch_bounds__(NTHR) void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
-  __shared__ CR<R> As[BK][BM + PAD];
-  __shared__ CR<R> Bs[BK][BN + PAD];
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(2)))
+void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
+  __shared__ CR<R> As2[2][BK][BM + PAD];
+  __shared__ CR<R> Bs2[2][BK][BN + PAD];
 
   const TileMap tm = map_tile(tiles_m, tiles_n, g.inner);
   const int tid = threadIdx.x;
@@ -131,83 +138,126 @@ __global__ __launch_bounds__(NTHR) void gemm_kernel(GemmArgs g, int tiles_m, int
   else           { b_c = tid >> 2; b_r = (tid & 3) * 4; }
 
   cplx ra[4], rb[4];
+  bool oka[4], okb[4];
 
+  // Branch-free tile loads: an out-of-range element is read from (0, 0), which
+  // always exists, and zeroed when staged into LDS.  All eight loads of a
+  // chunk are then in flight together instead of each behind its own branch.
   auto load_tiles = [&](int k0) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      int m = tm.m0 + a_r + (a_kcontig ? 0 : e);
-      int k = k0 + a_c + (a_kcontig ? e : 0);
-      cplx v{0.0, 0.0};
-      if (m < g.M && k < g.K) {
-        v = a_kcontig ? ld_c(A + (long long)m * g.lda + k) : ld_c(A + (long long)k * g.lda + m);
-        v.im *= a_conj;
-      }
-      ra[e] = v;
+      const int m = tm.m0 + a_r + (a_kcontig ? 0 : e);
+      const int k = k0 + a_c + (a_kcontig ? e : 0);
+      oka[e] = (m < g.M) && (k < g.K);
+      const long long mm = oka[e] ? m : 0, kk = oka[e] ? k : 0;
+      ra[e] = a_kcontig ? ld_c(A + mm * g.lda + kk) : ld_c(A + kk * g.lda + mm);
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      int k = k0 + b_r + (b_ncontig ? 0 : e);
-      int n = tm.n0 + b_c + (b_ncontig ? e : 0);
-      cplx v{0.0, 0.0};
-      if (k < g.K && n < g.N) {
-        v = b_ncontig ? ld_c(B + (long long)k * g.ldb + n) : ld_c(B + (long long)n * g.ldb + k);
-        v.im *= b_conj;
-      }
-      rb[e] = v;
+      const int k = k0 + b_r + (b_ncontig ? 0 : e);
+      const int n = tm.n0 + b_c + (b_ncontig ? e : 0);
+      okb[e] = (k < g.K) && (n < g.N);
+      const long long kk = okb[e] ? k : 0, nn = okb[e] ? n : 0;
+      rb[e] = b_ncontig ? ld_c(B + kk * g.ldb + nn) : ld_c(B + nn * g.ldb + kk);
     }
   };
-  auto store_tiles = [&]() {
+  auto store_tiles = [&](int buf) {
+    auto& As = As2[buf];
+    auto& Bs = Bs2[buf];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int m = a_r + (a_kcontig ? 0 : e);
       const int k = a_c + (a_kcontig ? e : 0);
-      As[k][m] = CR<R>{(R)ra[e].re, (R)ra[e].im};
+      const double re = oka[e] ? ra[e].re : 0.0, im = oka[e] ? ra[e].im * a_conj : 0.0;
+      As[k][m] = CR<R>{(R)re, (R)im};
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int k = b_r + (b_ncontig ? 0 : e);
       const int n = b_c + (b_ncontig ? e : 0);
-      Bs[k][n] = CR<R>{(R)rb[e].re, (R)rb[e].im};
+      const double re = okb[e] ? rb[e].re : 0.0, im = okb[e] ? rb[e].im * b_conj : 0.0;
+      Bs[k][n] = CR<R>{(R)re, (R)im};
     }
   };
 
   using V = typename Acc<R>::V;
-  V cre[2][2], cim[2][2];
+  // 3M (Karatsuba) complex product: T1 = Ar Br, T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi);
+  // Re = T1 - T2, Im = T3 - T1 - T2.  Three real MFMAs per complex product
+  // instead of four, normwise error of the same order as the 4-product form.
+  V t1[2][2], t2[2][2], t3[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      cre[i][j] = V{0, 0, 0, 0};
-      cim[i][j] = V{0, 0, 0, 0};
+      t1[i][j] = V{0, 0, 0, 0};
+      t2[i][j] = V{0, 0, 0, 0};
+      t3[i][j] = V{0, 0, 0, 0};
     }
 
-  const int ktiles = (g.K + BK - 1) / BK;
-  load_tiles(0);
-  for (int kt = 0; kt < ktiles; ++kt) {
-    store_tiles();
-    __syncthreads();
-    if (kt + 1 < ktiles) load_tiles((kt + 1) * BK);
+  // one LDS buffer is read by the MFMAs while the next chunk is staged into
+  // the other: a single barrier per chunk
+  auto compute = [&](int buf) {
+    auto& As = As2[buf];
+    auto& Bs = Bs2[buf];
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kr = kk + (lane >> 4);
       CR<R> a[2], b[2];
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        a[s] = As[kr][wm * 32 + s * 16 + (lane & 15)];
-        b[s] = Bs[kr][wn * 32 + s * 16 + (lane & 15)];
+      for (int q = 0; q < 2; ++q) {
+        a[q] = As[kr][wm * 32 + q * 16 + (lane & 15)];
+        b[q] = Bs[kr][wn * 32 + q * 16 + (lane & 15)];
       }
+      if constexpr (kThreeM<R>) {
+        R as[2], bs[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const R nai = -a[i].im;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          cre[i][j] = mfma(a[i].re, b[j].re, cre[i][j]);
-          cre[i][j] = mfma(nai, b[j].im, cre[i][j]);
-          cim[i][j] = mfma(a[i].re, b[j].im, cim[i][j]);
-          cim[i][j] = mfma(a[i].im, b[j].re, cim[i][j]);
+        for (int q = 0; q < 2; ++q) {
+          as[q] = a[q].re + a[q].im;
+          bs[q] = b[q].re + b[q].im;
         }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) t1[i][j] = mfma(a[i].re, b[j].re, t1[i][j]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) t2[i][j] = mfma(a[i].im, b[j].im, t2[i][j]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) t3[i][j] = mfma(as[i], bs[j], t3[i][j]);
+      } else {
+        // 4 real products: t1 = Re, t3 = Im (t2 unused)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) t1[i][j] = mfma(a[i].re, b[j].re, t1[i][j]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) t3[i][j] = mfma(a[i].re, b[j].im, t3[i][j]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) t1[i][j] = mfma(-a[i].im, b[j].im, t1[i][j]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) t3[i][j] = mfma(a[i].im, b[j].re, t3[i][j]);
       }
     }
+  };
+
+  const int ktiles = (g.K + BK - 1) / BK;
+  load_tiles(0);
+  store_tiles(0);
+  __syncthreads();
+  for (int kt = 0; kt < ktiles; ++kt) {
+    const bool more = kt + 1 < ktiles;
+    if (more) load_tiles((kt + 1) * BK);
+    compute(kt & 1);
+    if (more) store_tiles((kt + 1) & 1);
     __syncthreads();
   }
 
@@ -227,7 +277,15 @@ __global__ __launch_bounds__(NTHR) void gemm_kernel(GemmArgs g, int tiles_m, int
         const int row = tm.m0 + wm * 32 + i * 16 + acc_row<R>(lane, v);
         const int col = tm.n0 + wn * 32 + j * 16 + (lane & 15);
         if (row < g.M && col < g.N) {
-          const double xr = (double)cre[i][j][v], xi = (double)cim[i][j][v];
+          double xr, xi;
+          if constexpr (kThreeM<R>) {
+            const double p1 = (double)t1[i][j][v], p2 = (double)t2[i][j][v];
+            xr = p1 - p2;
+            xi = (double)t3[i][j][v] - p1 - p2;
+          } else {
+            xr = (double)t1[i][j][v];
+            xi = (double)t3[i][j][v];
+          }
           double outr = g.alpha.re * xr - g.alpha.im * xi;
           double outi = g.alpha.re * xi + g.alpha.im * xr;
           if (use_c0) {

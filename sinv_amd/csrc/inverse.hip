@@ -132,9 +132,13 @@ __global__ __launch_bounds__(256) void inv_leaf_kernel(cplx* A, int n, int lda, 
     }
 }
 
+// Split so that the recursion's GEMM dimensions stay multiples of the 64-wide
+// GEMM tile where possible: the largest multiple of 64 not above n/2, else a
+// multiple of 16 near n/2.
 int split_point(int n) {
-  int n1 = ((n / 2 + 15) / 16) * 16;
-  if (n1 >= n) n1 = n / 2;
+  int n1 = (n / 2) / 64 * 64;
+  if (n1 == 0) n1 = ((n / 2 + 15) / 16) * 16;
+  if (n1 >= n || n1 <= 0) n1 = n / 2;
   return n1;
 }
 
