@@ -80,6 +80,9 @@ __global__ __launch_bounds__(256) void inv_leaf_kernel(cplx* A, int n, int lda, 
           idx = oi;
         }
       }
+      // No comparable candidate (every entry NaN, or a lane >= n won the
+      // reduction): keep row k, so the pivot is always a row in [k, n).
+      if (idx < k || idx >= n) idx = k;
       if (tid == 0) {
         sp = idx;
         piv[k] = idx;

@@ -90,3 +90,15 @@ def test_zinv_singular_raises(built_lib):
     with pytest.raises(np.linalg.LinAlgError) as ei:
         core.zinv_batched(A)
     assert ei.value.instance == 1
+
+
+def test_zinv_nan_input_propagates(built_lib):
+    """NaN entries propagate to NaN results without a fault (no comparable
+    pivot candidate keeps the current row)."""
+    from sinv_amd import core
+    A = np.tile(np.eye(8, dtype=complex), (3, 1, 1))
+    A[1, 2, 3] = np.nan
+    A[2] = np.nan
+    X = core.zinv_batched(A, check=False).cpu().numpy()
+    assert np.allclose(X[0], np.eye(8))
+    assert np.isnan(X[1]).any() and np.isnan(X[2]).all()
