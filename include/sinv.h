@@ -103,7 +103,8 @@ int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_bytes, void
  * the map computed by rgf.py:40-62.  Given grad_G on the block-tridiagonal
  * pattern (torch's convention for a real loss: dL/dRe + i dL/dIm), returns
  * grad_A = -G^H grad_G G^H restricted to the pattern, for the same instance
- * convention as sinv_rgf_desc (uniform block size).  For
+ * convention as sinv_rgf_desc (uniform `blocksize`, or per-block sizes in
+ * `bsizes` with the packed layout described there).  For
  * A(E) = (E + i eta) I - H the gradient of H is -grad_A.
  * checkpoint_every = k > 0 stores the left-connected blocks only every k
  * blocks and recomputes each k-block segment during the backward sweep
@@ -112,6 +113,7 @@ int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_bytes, void
 typedef struct sinv_rgf_adjoint_desc {
   int nblocks;
   int blocksize;
+  const int* bsizes;
   int batch;
   int precision;
   int checkpoint_every;

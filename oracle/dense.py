@@ -69,3 +69,31 @@ def residuals(diag, upper, lower, Gd, Gu, Gl):
             C = C + Gu[i] @ lower[i]
         worst = max(worst, np.linalg.norm(R) / scale, np.linalg.norm(C) / scale)
     return worst
+
+
+def assemble_lists(diag, upper, lower):
+    """Dense matrix of a block-tridiagonal matrix with per-block sizes."""
+    bs = [d.shape[0] for d in diag]
+    off = np.concatenate([[0], np.cumsum(bs)])
+    A = np.zeros((off[-1], off[-1]), dtype=np.result_type(*diag))
+    for i in range(len(bs)):
+        A[off[i]:off[i + 1], off[i]:off[i + 1]] = diag[i]
+        if i < len(bs) - 1:
+            A[off[i]:off[i + 1], off[i + 1]:off[i + 2]] = upper[i]
+            A[off[i + 1]:off[i + 2], off[i]:off[i + 1]] = lower[i]
+    return A
+
+
+def select_lists(M, bs):
+    off = np.concatenate([[0], np.cumsum(bs)])
+    n = len(bs)
+    d = [M[off[i]:off[i + 1], off[i]:off[i + 1]] for i in range(n)]
+    u = [M[off[i]:off[i + 1], off[i + 1]:off[i + 2]] for i in range(n - 1)]
+    l = [M[off[i + 1]:off[i + 2], off[i]:off[i + 1]] for i in range(n - 1)]
+    return d, u, l
+
+
+def rel_err_lists(G, R):
+    num = sum(np.linalg.norm(a - r) ** 2 for ga, gr in zip(G, R) for a, r in zip(ga, gr))
+    den = sum(np.linalg.norm(r) ** 2 for gr in R for r in gr)
+    return float(np.sqrt(num / den))

@@ -16,7 +16,8 @@ matrices_utils.py:141-151 (``diag (N,b,b)``, ``upper[i] = A_{i,i+1}``,
       G_ii = g_i + (g_i @ A_{i,i+1}) @ X
     with the two off-diagonal writes skipped when ``save_off_diag`` is False.
 
-``rgf_blocks`` is the restatement.
+``rgf_blocks`` is the restatement; ``rgf_lists`` runs the same recurrences
+on lists of blocks of different sizes (b_i per block row).
 """
 
 import numpy as np
@@ -43,5 +44,25 @@ def rgf_blocks(diag, upper, lower, sym_mat=False, save_off_diag=True):
                 Gu[i] = Gl[i].T
             else:
                 Gu[i] = -g_ii @ upper[i] @ Gd[i + 1]
+        Gd[i] = g_ii + g_ii @ upper[i] @ X
+    return Gd, Gu, Gl
+
+
+def rgf_lists(diag, upper, lower, sym_mat=False, save_off_diag=True):
+    """rgf.py:40-62 on lists: diag[i] b_i x b_i, upper[i] b_i x b_{i+1},
+    lower[i] b_{i+1} x b_i."""
+    N = len(diag)
+    Gd = [None] * N
+    Gu = [np.full_like(u, np.nan) for u in upper]
+    Gl = [np.full_like(l, np.nan) for l in lower]
+    Gd[0] = inv(diag[0])
+    for i in range(1, N):
+        Gd[i] = inv(diag[i] - lower[i - 1] @ Gd[i - 1] @ upper[i - 1])
+    for i in range(N - 2, -1, -1):
+        g_ii = Gd[i].copy()
+        X = Gd[i + 1] @ lower[i] @ g_ii
+        if save_off_diag:
+            Gl[i] = -X
+            Gu[i] = Gl[i].T if sym_mat else -g_ii @ upper[i] @ Gd[i + 1]
         Gd[i] = g_ii + g_ii @ upper[i] @ X
     return Gd, Gu, Gl

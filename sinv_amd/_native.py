@@ -65,7 +65,8 @@ class BcrDesc(ctypes.Structure):
 
 class AdjDesc(ctypes.Structure):
     _fields_ = [
-        ("nblocks", _i), ("blocksize", _i), ("batch", _i), ("precision", _i),
+        ("nblocks", _i), ("blocksize", _i), ("bsizes", ctypes.POINTER(_i)), ("batch", _i),
+        ("precision", _i),
         ("checkpoint_every", _i),
         ("diag", _vp), ("diag_stride", _ll),
         ("upper", _vp), ("upper_stride", _ll),

@@ -170,6 +170,23 @@ class RgfPlan:
     def outputs(self):
         return self.Gd, self.Gu, self.Gl
 
+    def capture(self, warmup=1):
+        """Capture the whole launch sequence of one ``run()`` into a HIP graph
+        (torch.cuda.CUDAGraph over the HIP runtime).  Returns the graph; its
+        ``replay()`` re-runs every kernel of the sweep with one launch call.
+        The sequence holds kernels only (the info word is cleared by a fill
+        kernel, not a memset), so replays are exact repeats."""
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for _ in range(warmup):
+                self.run()
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run()
+        return g
+
 
 def bcr_batched(diag, upper, lower, *, shift=None, dscale=1.0, oscale=1.0, return_fill=False,
                 check=True):
@@ -264,6 +281,135 @@ def rgf_adjoint_batched(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *
     if check:
         nat.check_info(info)
     return Ad, Au, Al
+
+
+# ---------------------------------------------------------------------------
+# ragged block sizes (per-block b_i): lists of blocks, packed back to back on
+# the device in the layout documented with sinv_rgf_desc.bsizes
+
+
+def _pack(blocks):
+    """List of (b_r x b_c) or (B, b_r, b_c) blocks -> (packed tensor, batch or
+    None, instance stride)."""
+    ts = [to_dev(b) for b in blocks]
+    Bs = {t.shape[0] for t in ts if t.dim() == 3}
+    if len(Bs) > 1:
+        raise ValueError(f"inconsistent batch sizes {sorted(Bs)}")
+    if not ts:
+        return None, None, 0
+    if Bs:
+        B = Bs.pop()
+        flat = [(t if t.dim() == 3 else t.expand(B, *t.shape)).reshape(B, -1) for t in ts]
+        packed = torch.cat(flat, dim=1).contiguous()
+        return packed, B, packed.shape[1]
+    return torch.cat([t.reshape(-1) for t in ts]).contiguous(), None, 0
+
+
+def _unpack(packed, shapes):
+    out, o = [], 0
+    for r, c in shapes:
+        out.append(packed[:, o:o + r * c].view(packed.shape[0], r, c))
+        o += r * c
+    return out
+
+
+def _ragged_sizes(diag, upper, lower):
+    bs = [int(np.shape(d)[-1]) for d in diag]
+    N = len(bs)
+    for i, d in enumerate(diag):
+        if tuple(np.shape(d)[-2:]) != (bs[i], bs[i]):
+            raise ValueError(f"diagonal block {i} is not square")
+    if len(upper) != N - 1 or len(lower) != N - 1:
+        raise ValueError("need N-1 upper and lower blocks")
+    for i in range(N - 1):
+        if tuple(np.shape(upper[i])[-2:]) != (bs[i], bs[i + 1]):
+            raise ValueError(f"upper block {i} must be {bs[i]} x {bs[i + 1]}")
+        if tuple(np.shape(lower[i])[-2:]) != (bs[i + 1], bs[i]):
+            raise ValueError(f"lower block {i} must be {bs[i + 1]} x {bs[i]}")
+    return bs
+
+
+def rgf_ragged(diag, upper, lower, *, sym_mat=False, save_off_diag=True, precision="fp64",
+               shift=None, dscale=1.0, oscale=1.0, check=True):
+    """RGF with per-block sizes: diag[i] is b_i x b_i, upper[i] b_i x b_{i+1},
+    lower[i] b_{i+1} x b_i (each optionally with a leading batch dimension).
+    Returns lists of device tensors (B, ...) for the diagonal, upper and lower
+    blocks of G."""
+    if precision not in nat.PREC:
+        raise ValueError(f"precision must be one of {sorted(nat.PREC)}")
+    bs = _ragged_sizes(diag, upper, lower)
+    N = len(bs)
+    d, Bd, sd = _pack(diag)
+    u, Bu, su = _pack(upper)
+    l, Bl, sl = _pack(lower)
+    sh = to_dev(shift).reshape(-1) if shift is not None else None
+    B = _resolve_batch(Bd, Bu, Bl, None if sh is None else sh.numel())
+    dsize = sum(b * b for b in bs)
+    osize = sum(bs[i] * bs[i + 1] for i in range(N - 1))
+    dev = device()
+    Gd = torch.empty((B, dsize), dtype=_c128, device=dev)
+    Gu = torch.empty((B, max(osize, 1)), dtype=_c128, device=dev)
+    Gl = torch.empty((B, max(osize, 1)), dtype=_c128, device=dev)
+    info = torch.empty(1, dtype=torch.int64, device=dev)
+    bsz = (ctypes.c_int * N)(*bs)
+    desc = nat.RgfDesc(
+        nblocks=N, blocksize=0, bsizes=bsz, batch=B, sym_mat=int(bool(sym_mat)),
+        save_off_diag=int(bool(save_off_diag)), precision=nat.PREC[precision],
+        diag=_p(d), diag_stride=sd, upper=_p(u), upper_stride=su, lower=_p(l), lower_stride=sl,
+        dscale=nat.C128.of(dscale), oscale=nat.C128.of(oscale), shift=_p(sh),
+        g_diag=_p(Gd), g_upper=_p(Gu), g_lower=_p(Gl), info=_info_ptr(info))
+    L = nat.lib()
+    ws = _workspace(L.sinv_rgf_workspace_bytes(ctypes.byref(desc)))
+    nat.check(L.sinv_rgf_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()), "rgf")
+    if check:
+        nat.check_info(info)
+    return (_unpack(Gd, [(b, b) for b in bs]),
+            _unpack(Gu, [(bs[i], bs[i + 1]) for i in range(N - 1)]),
+            _unpack(Gl, [(bs[i + 1], bs[i]) for i in range(N - 1)]))
+
+
+def rgf_adjoint_ragged(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *, shift=None,
+                       dscale=1.0, oscale=1.0, precision="fp64", checkpoint_every=0, check=True):
+    """``rgf_adjoint_batched`` with per-block sizes (lists of blocks, as
+    ``rgf_ragged``).  Returns lists of gradient blocks (B, ...)."""
+    if precision not in nat.PREC:
+        raise ValueError(f"precision must be one of {sorted(nat.PREC)}")
+    bs = _ragged_sizes(diag, upper, lower)
+    _ragged_sizes(gbar_diag, gbar_upper, gbar_lower)
+    N = len(bs)
+    d, Bd, sd = _pack(diag)
+    u, Bu, su = _pack(upper)
+    l, Bl, sl = _pack(lower)
+    gd, Bgd, sgd = _pack(gbar_diag)
+    gu, Bgu, sgu = _pack(gbar_upper)
+    gl, Bgl, sgl = _pack(gbar_lower)
+    sh = to_dev(shift).reshape(-1) if shift is not None else None
+    B = _resolve_batch(Bd, Bu, Bl, Bgd, Bgu, Bgl, None if sh is None else sh.numel())
+    dsize = sum(b * b for b in bs)
+    osize = sum(bs[i] * bs[i + 1] for i in range(N - 1))
+    dev = device()
+    Ad = torch.empty((B, dsize), dtype=_c128, device=dev)
+    Au = torch.empty((B, max(osize, 1)), dtype=_c128, device=dev)
+    Al = torch.empty((B, max(osize, 1)), dtype=_c128, device=dev)
+    info = torch.empty(1, dtype=torch.int64, device=dev)
+    bsz = (ctypes.c_int * N)(*bs)
+    desc = nat.AdjDesc(
+        nblocks=N, blocksize=0, bsizes=bsz, batch=B, precision=nat.PREC[precision],
+        checkpoint_every=int(checkpoint_every),
+        diag=_p(d), diag_stride=sd, upper=_p(u), upper_stride=su, lower=_p(l), lower_stride=sl,
+        dscale=nat.C128.of(dscale), oscale=nat.C128.of(oscale), shift=_p(sh),
+        gbar_diag=_p(gd), gbar_diag_stride=sgd, gbar_upper=_p(gu), gbar_upper_stride=sgu,
+        gbar_lower=_p(gl), gbar_lower_stride=sgl,
+        abar_diag=_p(Ad), abar_upper=_p(Au), abar_lower=_p(Al), info=_info_ptr(info))
+    L = nat.lib()
+    ws = _workspace(L.sinv_rgf_adjoint_workspace_bytes(ctypes.byref(desc)))
+    nat.check(L.sinv_rgf_adjoint_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()),
+              "rgf_adjoint")
+    if check:
+        nat.check_info(info)
+    return (_unpack(Ad, [(b, b) for b in bs]),
+            _unpack(Au, [(bs[i], bs[i + 1]) for i in range(N - 1)]),
+            _unpack(Al, [(bs[i + 1], bs[i]) for i in range(N - 1)]))
 
 
 def bcr_fill_positions(N):

@@ -29,7 +29,8 @@ using namespace sinv;
 namespace {
 
 bool adj_desc_ok(const sinv_rgf_adjoint_desc* d) {
-  if (d == nullptr || d->nblocks <= 0 || d->blocksize <= 0 || d->batch <= 0) return false;
+  if (d == nullptr || d->nblocks <= 0 || d->batch <= 0) return false;
+  if (d->bsizes == nullptr && d->blocksize <= 0) return false;
   if (d->precision != SINV_PREC_FP64 && d->precision != SINV_PREC_MIXED) return false;
   if (d->diag == nullptr || d->gbar_diag == nullptr || d->abar_diag == nullptr) return false;
   if (d->nblocks > 1 && (d->upper == nullptr || d->lower == nullptr || d->gbar_upper == nullptr ||
@@ -51,16 +52,22 @@ struct AdjPlan {
 
 constexpr int kTemps = 12;
 
-AdjPlan adj_plan(int N, int b, int batch, int every) {
+AdjPlan adj_plan(const BlockLayout& L, int batch, int every) {
   AdjPlan P;
+  const int N = L.N;
   int k = every;
   if (k <= 0 || k >= N) k = N;
   if (k < 2) k = 2;  // the forward step reads slot i-1 while writing slot i
   P.k = k;
   P.nseg = (N + k - 1) / k;
   P.blocks = kTemps + 2LL * k + 2LL * (P.nseg > 1 ? P.nseg - 1 : 0);
-  P.inv_elems = inverse_workspace_elems(b, batch);
-  P.bytes = ((size_t)P.blocks * b * b * batch + P.inv_elems) * sizeof(cplx) + 512;
+  size_t inv = 0;
+  for (int i = 0; i < N; ++i) {
+    const size_t e = inverse_workspace_elems(L.b[i], batch);
+    if (e > inv) inv = e;
+  }
+  P.inv_elems = inv;
+  P.bytes = ((size_t)P.blocks * L.bmax * L.bmax * batch + P.inv_elems) * sizeof(cplx) + 512;
   return P;
 }
 
@@ -68,7 +75,9 @@ AdjPlan adj_plan(int N, int b, int batch, int every) {
 
 extern "C" size_t sinv_rgf_adjoint_workspace_bytes(const sinv_rgf_adjoint_desc* d) {
   if (!adj_desc_ok(d)) return 0;
-  return adj_plan(d->nblocks, d->blocksize, d->batch, d->checkpoint_every).bytes;
+  const BlockLayout L = make_block_layout(d->nblocks, d->blocksize, d->bsizes);
+  if (!L.ok) return 0;
+  return adj_plan(L, d->batch, d->checkpoint_every).bytes;
 }
 
 extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* work,
@@ -77,15 +86,20 @@ extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* wo
     set_error("sinv_rgf_adjoint_batched: invalid descriptor");
     return SINV_E_INVAL;
   }
-  const int N = d->nblocks, b = d->blocksize, nb = d->batch;
-  const AdjPlan P = adj_plan(N, b, nb, d->checkpoint_every);
+  const BlockLayout Ly = make_block_layout(d->nblocks, d->blocksize, d->bsizes);
+  if (!Ly.ok) {
+    set_error("sinv_rgf_adjoint_batched: invalid block sizes");
+    return SINV_E_INVAL;
+  }
+  const int N = Ly.N, nb = d->batch;
+  const AdjPlan P = adj_plan(Ly, nb, d->checkpoint_every);
   if (work == nullptr || work_bytes < P.bytes) {
     set_error("sinv_rgf_adjoint_batched: workspace too small (%zu < %zu)", work_bytes, P.bytes);
     return SINV_E_INVAL;
   }
   hipStream_t s = (hipStream_t)stream;
   const Prec prec = d->precision == SINV_PREC_FP64 ? Prec::F64 : Prec::F32;
-  const long long b2 = (long long)b * b;
+  const long long b2 = (long long)Ly.bmax * Ly.bmax;  // slot size
   cplx* base = (cplx*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
   const long long WI = P.blocks * b2;  // instance stride in the workspace
   cplx* winv = base + WI * nb;
@@ -116,70 +130,76 @@ extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* wo
   cplx* Ad = (cplx*)d->abar_diag;
   cplx* Au = (cplx*)d->abar_upper;
   cplx* Al = (cplx*)d->abar_lower;
-  const long long AdI = (long long)N * b2, AoI = (long long)(N > 1 ? N - 1 : 0) * b2;
+  const long long AdI = Ly.dsize, AoI = Ly.osize;
   const cplx one{1.0, 0.0}, mone{-1.0, 0.0}, zero{0.0, 0.0};
   const cplx cds{d->dscale.re, -d->dscale.im};  // conj(dscale)
   const cplx cos_{d->oscale.re, -d->oscale.im};  // conj(oscale)
   const cplx mcos{-cos_.re, -cos_.im};
+  auto bs = [&](int i) { return Ly.b[i]; };
 
-  // operand descriptor: pointer of instance 0, instance stride, op
+  // operand: pointer of instance 0, instance stride, op, leading dimension
   struct Opd {
     const cplx* p;
     long long inst;
     int op;
+    int ld;
   };
-  auto ws = [&](const cplx* p) { return Opd{p, WI, OP_N}; };
-  auto gemm = [&](Opd A, Opd B, cplx* C, long long cinst, cplx alpha, const cplx* C0,
-                  long long c0inst) -> hipError_t {
+  auto ws = [&](const cplx* p, int ld) { return Opd{p, WI, OP_N, ld}; };
+  // C (M x N, ldc) = alpha op(A) op(B) (+ C0 when given), contraction length K
+  auto gemm = [&](int M, int Nn, int K, Opd A, Opd B, cplx* C, long long cinst, int ldc,
+                  cplx alpha, const cplx* C0, long long c0inst) -> hipError_t {
     GemmArgs g{};
-    g.M = b; g.N = b; g.K = b; g.opA = A.op; g.opB = B.op;
-    g.A = A.p; g.lda = b; g.sA_in = 0; g.sA_out = A.inst;
-    g.B = B.p; g.ldb = b; g.sB_in = 0; g.sB_out = B.inst;
-    g.C = C; g.ldc = b; g.sC_in = 0; g.sC_out = cinst;
-    g.C0 = C0; g.ldc0 = b; g.sC0_in = 0; g.sC0_out = c0inst;
+    g.M = M; g.N = Nn; g.K = K; g.opA = A.op; g.opB = B.op;
+    g.A = A.p; g.lda = A.ld; g.sA_in = 0; g.sA_out = A.inst;
+    g.B = B.p; g.ldb = B.ld; g.sB_in = 0; g.sB_out = B.inst;
+    g.C = C; g.ldc = ldc; g.sC_in = 0; g.sC_out = cinst;
+    g.C0 = C0; g.ldc0 = ldc; g.sC0_in = 0; g.sC0_out = c0inst;
     g.alpha = alpha;
     g.beta = C0 ? one : zero;
     g.shift = nullptr;
     g.inner = 1; g.outer = nb;
     return launch_gemm(g, prec, s);
   };
-  // C = alpha A B (+ C when acc)
-  auto mm = [&](Opd A, Opd B, cplx* C, long long cinst, cplx alpha, bool acc) {
-    return gemm(A, B, C, cinst, alpha, acc ? C : nullptr, cinst);
+  auto mm = [&](int M, int Nn, int K, Opd A, Opd B, cplx* C, long long cinst, int ldc, cplx alpha,
+                bool acc) { return gemm(M, Nn, K, A, B, C, cinst, ldc, alpha, acc ? C : nullptr, cinst); };
+  auto copy = [&](int r, int c, const cplx* src, long long sinst, cplx* dst, long long dinst) {
+    return launch_axpby_shift(r, c, one, src, c, 0, sinst, dst, c, 0, dinst, nullptr, 1, nb, s);
   };
-  auto copy = [&](const cplx* src, long long sinst, cplx* dst, long long dinst) {
-    return launch_axpby_shift(b, b, one, src, b, 0, sinst, dst, b, 0, dinst, nullptr, 1, nb, s);
-  };
-  // blocks of A / grad_G
-  auto Ublk = [&](int i) { return Opd{U + i * b2, d->upper_stride, OP_C}; };   // -> B_{i+1,i}
-  auto Lblk = [&](int i) { return Opd{Lo + i * b2, d->lower_stride, OP_C}; };  // -> B_{i,i+1}
-  auto Tdb = [&](int i) { return Opd{Td + i * b2, d->gbar_diag_stride, OP_N}; };
-  auto Tub = [&](int i) { return Opd{Tu + i * b2, d->gbar_upper_stride, OP_N}; };
-  auto Tlb = [&](int i) { return Opd{Tl + i * b2, d->gbar_lower_stride, OP_N}; };
+  // blocks of A (read conjugate-transposed as blocks of B = A^H) and of grad_G
+  //   U[i] = A_{i,i+1} (b_i x b_{i+1});  U[i]^H = B_{i+1,i}
+  //   Lo[i] = A_{i+1,i} (b_{i+1} x b_i); Lo[i]^H = B_{i,i+1}
+  auto Ublk = [&](int i) { return Opd{U + Ly.ooff[i], d->upper_stride, OP_C, bs(i + 1)}; };
+  auto Lblk = [&](int i) { return Opd{Lo + Ly.ooff[i], d->lower_stride, OP_C, bs(i)}; };
+  auto Tdb = [&](int i) { return Opd{Td + Ly.doff[i], d->gbar_diag_stride, OP_N, bs(i)}; };
+  auto Tub = [&](int i) { return Opd{Tu + Ly.ooff[i], d->gbar_upper_stride, OP_N, bs(i + 1)}; };
+  auto Tlb = [&](int i) { return Opd{Tl + Ly.ooff[i], d->gbar_lower_stride, OP_N, bs(i)}; };
 
   // forward step i of RGF on B with tangents: (g_prev, dg_prev) -> (g_out, dg_out)
   auto fwd_step = [&](int i, cplx* gp, cplx* dgp, cplx* go, cplx* dgo) -> hipError_t {
+    const int bi = bs(i);
     // B_ii = conj(dscale) D_ii^H + conj(shift) I
     cplx* bdst = i == 0 ? go : Bd;
-    SINV_HIP_RET(launch_transpose(b, b, OP_C, D + i * b2, b, 0, d->diag_stride, bdst, b, 0, WI,
-                                  1, nb, s, cds, (const cplx*)d->shift, true));
+    SINV_HIP_RET(launch_transpose(bi, bi, OP_C, D + Ly.doff[i], bi, 0, d->diag_stride, bdst, bi,
+                                  0, WI, 1, nb, s, cds, (const cplx*)d->shift, true));
     if (i == 0) {
-      SINV_HIP_RET(launch_inverse(go, b, b, 0, WI, 1, nb, winv, d->info, 0, s));
-      SINV_HIP_RET(mm(ws(go), Tdb(0), Wt, WI, one, false));
-      return mm(ws(Wt), ws(go), dgo, WI, mone, false);
+      SINV_HIP_RET(launch_inverse(go, bi, bi, 0, WI, 1, nb, winv, d->info, 0, s));
+      SINV_HIP_RET(mm(bi, bi, bi, ws(go, bi), Tdb(0), Wt, WI, bi, one, false));
+      return mm(bi, bi, bi, ws(Wt, bi), ws(go, bi), dgo, WI, bi, mone, false);
     }
-    // Y = B_{i,i-1} g_{i-1};  M = B_ii - Y B_{i-1,i}
-    SINV_HIP_RET(mm(Ublk(i - 1), ws(gp), Y, WI, cos_, false));
-    SINV_HIP_RET(gemm(ws(Y), Lblk(i - 1), go, WI, mcos, Bd, WI));
+    const int bp = bs(i - 1);
+    // Y = B_{i,i-1} g_{i-1}  (b_i x b_{i-1});  M = B_ii - Y B_{i-1,i}
+    SINV_HIP_RET(mm(bi, bp, bp, Ublk(i - 1), ws(gp, bp), Y, WI, bp, cos_, false));
+    SINV_HIP_RET(gemm(bi, bi, bp, ws(Y, bp), Lblk(i - 1), go, WI, bi, mcos, Bd, WI));
     // dM = T_ii - (T_{i,i-1} g_{i-1} + B_{i,i-1} dg_{i-1}) B_{i-1,i} - Y T_{i-1,i}
-    SINV_HIP_RET(mm(Tlb(i - 1), ws(gp), Qf, WI, one, false));
-    SINV_HIP_RET(mm(Ublk(i - 1), ws(dgp), Qf, WI, cos_, true));
-    SINV_HIP_RET(gemm(ws(Qf), Lblk(i - 1), dM, WI, mcos, Td + i * b2, d->gbar_diag_stride));
-    SINV_HIP_RET(mm(ws(Y), Tub(i - 1), dM, WI, mone, true));
+    SINV_HIP_RET(mm(bi, bp, bp, Tlb(i - 1), ws(gp, bp), Qf, WI, bp, one, false));
+    SINV_HIP_RET(mm(bi, bp, bp, Ublk(i - 1), ws(dgp, bp), Qf, WI, bp, cos_, true));
+    SINV_HIP_RET(gemm(bi, bi, bp, ws(Qf, bp), Lblk(i - 1), dM, WI, bi, mcos, Td + Ly.doff[i],
+                      d->gbar_diag_stride));
+    SINV_HIP_RET(mm(bi, bi, bp, ws(Y, bp), Tub(i - 1), dM, WI, bi, mone, true));
     // g_i = M^-1, dg_i = -g_i dM g_i
-    SINV_HIP_RET(launch_inverse(go, b, b, 0, WI, 1, nb, winv, d->info, i, s));
-    SINV_HIP_RET(mm(ws(go), ws(dM), Wt, WI, one, false));
-    return mm(ws(Wt), ws(go), dgo, WI, mone, false);
+    SINV_HIP_RET(launch_inverse(go, bi, bi, 0, WI, 1, nb, winv, d->info, i, s));
+    SINV_HIP_RET(mm(bi, bi, bi, ws(go, bi), ws(dM, bi), Wt, WI, bi, one, false));
+    return mm(bi, bi, bi, ws(Wt, bi), ws(go, bi), dgo, WI, bi, mone, false);
   };
 
   if (d->info != nullptr) SINV_HIP_TRY(launch_fill_u64(d->info, 1, SINV_INFO_OK, s));
@@ -191,15 +211,16 @@ extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* wo
     SINV_HIP_TRY(fwd_step(i, gp, dgp, slot_g(i), slot_dg(i)));
     if (i % P.k == P.k - 1 && i + 1 < N) {
       const int seg = (i + 1) / P.k;
-      SINV_HIP_TRY(copy(slot_g(i), WI, ck_g(seg), WI));
-      SINV_HIP_TRY(copy(slot_dg(i), WI, ck_dg(seg), WI));
+      SINV_HIP_TRY(copy(bs(i), bs(i), slot_g(i), WI, ck_g(seg), WI));
+      SINV_HIP_TRY(copy(bs(i), bs(i), slot_dg(i), WI, ck_dg(seg), WI));
     }
   }
 
   // ---- backward sweep (rgf.py:51-62 on B, with tangents)
   int cur = 0;
-  SINV_HIP_TRY(copy(slot_g(N - 1), WI, Gpp[0], WI));               // G_{N-1,N-1}(B)
-  SINV_HIP_TRY(copy(slot_dg(N - 1), WI, Ad + (N - 1) * b2, AdI));  // its tangent
+  const int bl = bs(N - 1);
+  SINV_HIP_TRY(copy(bl, bl, slot_g(N - 1), WI, Gpp[0], WI));                  // G_{N-1,N-1}(B)
+  SINV_HIP_TRY(copy(bl, bl, slot_dg(N - 1), WI, Ad + Ly.doff[N - 1], AdI));  // its tangent
   for (int seg = P.nseg - 1; seg >= 0; --seg) {
     const int i0 = seg * P.k, i1 = (seg + 1) * P.k < N ? (seg + 1) * P.k : N;
     if (seg != P.nseg - 1) {
@@ -210,32 +231,33 @@ extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* wo
       }
     }
     for (int i = (i1 - 1 < N - 2 ? i1 - 1 : N - 2); i >= i0; --i) {
+      const int bi = bs(i), bj = bs(i + 1);
       cplx* g = slot_g(i);
       cplx* dg = slot_dg(i);
       cplx* Gn = Gpp[cur];
       cplx* Gi = Gpp[cur ^ 1];
-      const cplx* dGn = Ad + (i + 1) * b2;
-      cplx* Ali = Al + i * b2;
-      cplx* Aui = Au + i * b2;
-      cplx* Adi = Ad + i * b2;
+      const Opd dGn{Ad + Ly.doff[i + 1], AdI, OP_N, bj};
+      cplx* Ali = Al + Ly.ooff[i];  // b_{i+1} x b_i
+      cplx* Aui = Au + Ly.ooff[i];  // b_i x b_{i+1}
+      cplx* Adi = Ad + Ly.doff[i];  // b_i x b_i
       // primal: P = Gn B_{i+1,i}; Xn = -P g; Q = g B_{i,i+1}; G_ii = g - Q Xn
-      SINV_HIP_TRY(mm(ws(Gn), Ublk(i), Pb, WI, cos_, false));
-      SINV_HIP_TRY(mm(ws(Pb), ws(g), Xn, WI, mone, false));
-      SINV_HIP_TRY(mm(ws(g), Lblk(i), Qb, WI, cos_, false));
-      SINV_HIP_TRY(gemm(ws(Qb), ws(Xn), Gi, WI, mone, g, WI));
+      SINV_HIP_TRY(mm(bj, bi, bj, ws(Gn, bj), Ublk(i), Pb, WI, bi, cos_, false));
+      SINV_HIP_TRY(mm(bj, bi, bi, ws(Pb, bi), ws(g, bi), Xn, WI, bi, mone, false));
+      SINV_HIP_TRY(mm(bi, bj, bi, ws(g, bi), Lblk(i), Qb, WI, bj, cos_, false));
+      SINV_HIP_TRY(gemm(bi, bi, bj, ws(Qb, bj), ws(Xn, bi), Gi, WI, bi, mone, g, WI));
       // tangent of G_{i+1,i}: -(dGn B_{i+1,i} + Gn T_{i+1,i}) g - P dg
-      SINV_HIP_TRY(mm(Opd{dGn, AdI, OP_N}, Ublk(i), Rb, WI, cos_, false));
-      SINV_HIP_TRY(mm(ws(Gn), Tlb(i), Rb, WI, one, true));
-      SINV_HIP_TRY(mm(ws(Rb), ws(g), Ali, AoI, mone, false));
-      SINV_HIP_TRY(mm(ws(Pb), ws(dg), Ali, AoI, mone, true));
+      SINV_HIP_TRY(mm(bj, bi, bj, dGn, Ublk(i), Rb, WI, bi, cos_, false));
+      SINV_HIP_TRY(mm(bj, bi, bj, ws(Gn, bj), Tlb(i), Rb, WI, bi, one, true));
+      SINV_HIP_TRY(mm(bj, bi, bi, ws(Rb, bi), ws(g, bi), Ali, AoI, bi, mone, false));
+      SINV_HIP_TRY(mm(bj, bi, bi, ws(Pb, bi), ws(dg, bi), Ali, AoI, bi, mone, true));
       // tangent of G_{i,i+1}: -(dg B_{i,i+1} + g T_{i,i+1}) Gn - Q dGn
-      SINV_HIP_TRY(mm(ws(dg), Lblk(i), Sb, WI, cos_, false));
-      SINV_HIP_TRY(mm(ws(g), Tub(i), Sb, WI, one, true));
-      SINV_HIP_TRY(mm(ws(Sb), ws(Gn), Aui, AoI, mone, false));
-      SINV_HIP_TRY(mm(ws(Qb), Opd{dGn, AdI, OP_N}, Aui, AoI, mone, true));
+      SINV_HIP_TRY(mm(bi, bj, bi, ws(dg, bi), Lblk(i), Sb, WI, bj, cos_, false));
+      SINV_HIP_TRY(mm(bi, bj, bi, ws(g, bi), Tub(i), Sb, WI, bj, one, true));
+      SINV_HIP_TRY(mm(bi, bj, bj, ws(Sb, bj), ws(Gn, bj), Aui, AoI, bj, mone, false));
+      SINV_HIP_TRY(mm(bi, bj, bj, ws(Qb, bj), dGn, Aui, AoI, bj, mone, true));
       // tangent of G_ii: dg - S Xn - Q dXn
-      SINV_HIP_TRY(gemm(ws(Sb), ws(Xn), Adi, AdI, mone, dg, WI));
-      SINV_HIP_TRY(mm(ws(Qb), Opd{Ali, AoI, OP_N}, Adi, AdI, mone, true));
+      SINV_HIP_TRY(gemm(bi, bi, bj, ws(Sb, bj), ws(Xn, bi), Adi, AdI, bi, mone, dg, WI));
+      SINV_HIP_TRY(mm(bi, bi, bj, ws(Qb, bj), Opd{Ali, AoI, OP_N, bi}, Adi, AdI, bi, mone, true));
       cur ^= 1;
     }
   }

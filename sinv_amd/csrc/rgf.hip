@@ -25,40 +25,10 @@ namespace {
 
 thread_local char g_err[512] = "";
 
-struct Layout {
-  int N = 0;
-  std::vector<int> b;
-  std::vector<long long> doff, ooff;
-  long long dsize = 0, osize = 0;
-  int bmax = 0;
-  bool ok = false;
-};
+using Layout = BlockLayout;
 
 Layout make_layout(int N, int blocksize, const int* bsizes) {
-  Layout L;
-  if (N <= 0) return L;
-  L.N = N;
-  L.b.resize(N);
-  for (int i = 0; i < N; ++i) {
-    L.b[i] = bsizes ? bsizes[i] : blocksize;
-    if (L.b[i] <= 0) return L;
-    if (L.b[i] > L.bmax) L.bmax = L.b[i];
-  }
-  L.doff.resize(N);
-  L.ooff.resize(N > 1 ? N - 1 : 0);
-  long long d = 0, o = 0;
-  for (int i = 0; i < N; ++i) {
-    L.doff[i] = d;
-    d += (long long)L.b[i] * L.b[i];
-    if (i < N - 1) {
-      L.ooff[i] = o;
-      o += (long long)L.b[i] * L.b[i + 1];
-    }
-  }
-  L.dsize = d;
-  L.osize = o;
-  L.ok = true;
-  return L;
+  return make_block_layout(N, blocksize, bsizes);
 }
 
 struct WorkPlan {

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "../../include/sinv.h"
 
@@ -68,6 +69,45 @@ hipError_t launch_inverse(cplx* A, int n, int lda, long long sA_in, long long sA
 // Fill n elements with a value (used for NaN off-diagonals and info reset).
 hipError_t launch_fill(cplx* p, long long n, cplx v, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, long long n, unsigned long long v, hipStream_t s);
+
+// Packed block-tridiagonal layout with per-block sizes b_i: diagonal block i
+// at doff[i] (b_i x b_i), upper block i (b_i x b_{i+1}) and lower block i
+// (b_{i+1} x b_i) at ooff[i]; dsize / osize are the per-instance totals.
+struct BlockLayout {
+  int N = 0;
+  std::vector<int> b;
+  std::vector<long long> doff, ooff;
+  long long dsize = 0, osize = 0;
+  int bmax = 0;
+  bool ok = false;
+};
+
+inline BlockLayout make_block_layout(int N, int blocksize, const int* bsizes) {
+  BlockLayout L;
+  if (N <= 0) return L;
+  L.N = N;
+  L.b.resize(N);
+  for (int i = 0; i < N; ++i) {
+    L.b[i] = bsizes ? bsizes[i] : blocksize;
+    if (L.b[i] <= 0) return L;
+    if (L.b[i] > L.bmax) L.bmax = L.b[i];
+  }
+  L.doff.resize(N);
+  L.ooff.resize(N > 1 ? N - 1 : 0);
+  long long d = 0, o = 0;
+  for (int i = 0; i < N; ++i) {
+    L.doff[i] = d;
+    d += (long long)L.b[i] * L.b[i];
+    if (i < N - 1) {
+      L.ooff[i] = o;
+      o += (long long)L.b[i] * L.b[i + 1];
+    }
+  }
+  L.dsize = d;
+  L.osize = o;
+  L.ok = true;
+  return L;
+}
 
 // thread-local last error text for sinv_last_error()
 void set_error(const char* fmt, ...);

@@ -73,3 +73,26 @@ def rgf_flops_ragged(bs):
     f += sum(8.0 * (bs[i] * bs[i - 1] ** 2 + bs[i] ** 2 * bs[i - 1]) for i in range(1, N))
     f += sum(8.0 * (2 * bs[i] * bs[i + 1] ** 2 + 4 * bs[i] ** 2 * bs[i + 1]) for i in range(N - 1))
     return f
+
+
+def hermitian_H_ragged(bs, seed=0, coupling=0.5):
+    """Hermitian block-tridiagonal H with per-block sizes bs (lists of blocks):
+    re/im ~ N(0, 1/sqrt(b_i)) on the diagonal blocks, couplings ~ N(0,
+    1/sqrt(sqrt(b_i b_{i+1}))) scaled by `coupling`, H_{i+1,i} = H_{i,i+1}^H."""
+    rng = np.random.default_rng(seed)
+    diag, upper, lower = [], [], []
+    for i, b in enumerate(bs):
+        s = 1.0 / np.sqrt(b)
+        X = rng.normal(0, s, (b, b)) + 1j * rng.normal(0, s, (b, b))
+        diag.append(0.5 * (X + X.conj().T))
+    for i in range(len(bs) - 1):
+        s = 1.0 / np.sqrt(np.sqrt(bs[i] * bs[i + 1]))
+        U = coupling * (rng.normal(0, s, (bs[i], bs[i + 1])) + 1j * rng.normal(0, s, (bs[i], bs[i + 1])))
+        upper.append(U)
+        lower.append(np.ascontiguousarray(U.conj().T))
+    return diag, upper, lower
+
+
+def shifted_lists(H, E):
+    d, u, l = H
+    return [E * np.eye(x.shape[0]) - x for x in d], [-x for x in u], [-x for x in l]
