@@ -172,6 +172,37 @@ long long sinv_bcr_fill_blocks(int nblocks);
 int sinv_bcr_batched(const sinv_bcr_desc* d, void* work, size_t work_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
+ * PSR selected inversion on one GPU, batched over instances.
+ * Replaces psr_s.psr_seqsolve (src/sinv/algorithms/psr/psr_seqsolve.py:20-72)
+ * with `npart` partitions in place of MPI ranks: partition-local Schur
+ * reduction (:127-309) and production (:693-898) as batched kernels over
+ * partitions, the 2(npart-1)-block reduced system (:313-383, :470-491)
+ * solved by RGF.  Partitioning as psr_utils.divide_matrix (:61-99); needs
+ * npart >= 2 and nblocks >= 3 npart (psr_utils.check_input :44-55).
+ * Instance convention as sinv_rgf_desc (uniform block size).  Outputs the
+ * diagonal, upper and lower blocks of G.
+ * ------------------------------------------------------------------------ */
+typedef struct sinv_psr_desc {
+  int nblocks;
+  int blocksize;
+  int batch;
+  int npart;
+  const void* diag;  long long diag_stride;
+  const void* upper; long long upper_stride;
+  const void* lower; long long lower_stride;
+  sinv_c128 dscale;
+  sinv_c128 oscale;
+  const void* shift;
+  void* g_diag;
+  void* g_upper;
+  void* g_lower;
+  unsigned long long* info;
+} sinv_psr_desc;
+
+size_t sinv_psr_workspace_bytes(const sinv_psr_desc* d);
+int sinv_psr_batched(const sinv_psr_desc* d, void* work, size_t work_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
  * Batched in-place complex128 block inverse (kernel K1).
  * Replaces the numpy.linalg.inv calls of the hot path (rgf.py:44,48;
  * bcr_serial.py:121,148,249,302,379).  Blocks with n <= 64 use Gauss-Jordan

@@ -63,6 +63,19 @@ class BcrDesc(ctypes.Structure):
     ]
 
 
+class PsrDesc(ctypes.Structure):
+    _fields_ = [
+        ("nblocks", _i), ("blocksize", _i), ("batch", _i), ("npart", _i),
+        ("diag", _vp), ("diag_stride", _ll),
+        ("upper", _vp), ("upper_stride", _ll),
+        ("lower", _vp), ("lower_stride", _ll),
+        ("dscale", C128), ("oscale", C128),
+        ("shift", _vp),
+        ("g_diag", _vp), ("g_upper", _vp), ("g_lower", _vp),
+        ("info", ctypes.POINTER(ctypes.c_ulonglong)),
+    ]
+
+
 class AdjDesc(ctypes.Structure):
     _fields_ = [
         ("nblocks", _i), ("blocksize", _i), ("bsizes", ctypes.POINTER(_i)), ("batch", _i),
@@ -120,6 +133,11 @@ def lib():
         L.sinv_bcr_fill_blocks.argtypes = [_i]
         L.sinv_bcr_batched.restype = _i
         L.sinv_bcr_batched.argtypes = [ctypes.POINTER(BcrDesc), _vp, ctypes.c_size_t, _vp]
+    if hasattr(L, "sinv_psr_batched"):
+        L.sinv_psr_workspace_bytes.restype = ctypes.c_size_t
+        L.sinv_psr_workspace_bytes.argtypes = [ctypes.POINTER(PsrDesc)]
+        L.sinv_psr_batched.restype = _i
+        L.sinv_psr_batched.argtypes = [ctypes.POINTER(PsrDesc), _vp, ctypes.c_size_t, _vp]
     if hasattr(L, "sinv_rgf_adjoint_batched"):
         L.sinv_rgf_adjoint_workspace_bytes.restype = ctypes.c_size_t
         L.sinv_rgf_adjoint_workspace_bytes.argtypes = [ctypes.POINTER(AdjDesc)]

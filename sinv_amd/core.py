@@ -412,6 +412,39 @@ def rgf_adjoint_ragged(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *,
             _unpack(Al, [(bs[i + 1], bs[i]) for i in range(N - 1)]))
 
 
+def psr_batched(diag, upper, lower, npart, *, shift=None, dscale=1.0, oscale=1.0, check=True):
+    """PSR selected inversion (psr_seqsolve.py:20-72) with ``npart`` partitions
+    on one GPU: partition-local Schur reduction and production batched over
+    partitions, the 2(npart-1)-block reduced system solved by RGF.  Needs
+    nblocks >= 3 * npart (psr_utils.check_input).  Same instance convention and
+    outputs as ``rgf_batched``."""
+    d, Bd, sd = _batched(to_dev(diag), 3)
+    N, b = d.shape[-3], d.shape[-1]
+    if N < 3 * npart or npart < 2:
+        raise ValueError("The number of blocks is to low. There should be at least 3 blockrows "
+                         "per partition (and at least 2 partitions)")
+    u, Bu, su = _batched(to_dev(upper), 3)
+    l, Bl, sl = _batched(to_dev(lower), 3)
+    sh = to_dev(shift).reshape(-1) if shift is not None else None
+    B = _resolve_batch(Bd, Bu, Bl, None if sh is None else sh.numel())
+    dev = device()
+    Gd = torch.empty((B, N, b, b), dtype=_c128, device=dev)
+    Gu = torch.empty((B, N - 1, b, b), dtype=_c128, device=dev)
+    Gl = torch.empty((B, N - 1, b, b), dtype=_c128, device=dev)
+    info = torch.empty(1, dtype=torch.int64, device=dev)
+    desc = nat.PsrDesc(
+        nblocks=N, blocksize=b, batch=B, npart=int(npart), diag=_p(d), diag_stride=sd,
+        upper=_p(u), upper_stride=su, lower=_p(l), lower_stride=sl, dscale=nat.C128.of(dscale),
+        oscale=nat.C128.of(oscale), shift=_p(sh), g_diag=_p(Gd), g_upper=_p(Gu), g_lower=_p(Gl),
+        info=_info_ptr(info))
+    L = nat.lib()
+    ws = _workspace(L.sinv_psr_workspace_bytes(ctypes.byref(desc)))
+    nat.check(L.sinv_psr_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()), "psr")
+    if check:
+        nat.check_info(info)
+    return Gd, Gu, Gl
+
+
 def bcr_fill_positions(N):
     """(row, col) block positions of the blocks bcr_batched(return_fill=True) returns."""
     out = []

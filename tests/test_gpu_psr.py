@@ -1,0 +1,66 @@
+"""GPU parity of PSR on one GPU (psr_seqsolve.py:20-72, partitions in place of
+ranks) against the oracle's single-process restatement (oracle/psr_ref.py,
+pinned to the dense inverse in tests/test_oracle_psr.py).  Shapes and
+generator of psr_test.py:40-80 (SEED = 63) for 3..4 partitions, plus larger
+and batched cases."""
+
+import numpy as np
+import pytest
+
+from oracle import dense, psr_ref
+from oracle import matu_ref as mu
+
+pytestmark = pytest.mark.gpu
+
+SEED = 63
+PSR_SHAPES = [(9, 1), (18, 2), (27, 3), (12, 1), (24, 2), (36, 3), (256, 8), (240, 10), (144, 12)]
+
+
+@pytest.mark.parametrize("n,b", PSR_SHAPES)
+@pytest.mark.parametrize("P", [3, 4])
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("sym", [False, True])
+def test_psr_reference_shapes(built_lib, n, b, P, cplx, sym):
+    from sinv_amd import core
+    if n // b < 3 * P:
+        pytest.skip("fewer than 3 block rows per partition (psr_test.py:71 skips these too)")
+    print(f"psr n={n} b={b} P={P} cplx={cplx} sym={sym}", flush=True)
+    A = mu.generateBandedDiagonalMatrix(n, np.ceil(b / 2), cplx, sym, SEED)
+    d, u, l = mu.convertDenseToBlkTridiag(A, b)
+    got = tuple(x[0].cpu().numpy() for x in core.psr_batched(d, u, l, P))
+    ref = mu.convertDenseToBlkTridiag(np.linalg.inv(A), b)
+    for g, r in zip(got, ref):
+        assert np.allclose(g, r)
+    O = psr_ref.psr_blocks(d.astype(complex), u.astype(complex), l.astype(complex), P)
+    assert dense.rel_err(got, O) < 1e-15 * np.linalg.cond(A) * 10
+
+
+@pytest.mark.parametrize("N,b,P", [(12, 16, 2), (30, 24, 5), (64, 64, 8), (40, 96, 3)])
+def test_psr_dominant(built_lib, N, b, P):
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    d, u, l = configs.dominant_blocks(N, b, seed=N + P)
+    got = tuple(x[0].cpu().numpy() for x in core.psr_batched(d, u, l, P))
+    assert dense.rel_err(got, psr_ref.psr_blocks(d, u, l, P)) < 1e-12
+    assert dense.rel_err(got, dense.selected_inverse_dense(d, u, l)) < 1e-12
+
+
+def test_psr_energies_agrees_with_rgf(built_lib):
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    H = configs.hermitian_H(24, 48, seed=8)
+    E = configs.energies(3, seed=9) + 1e-2j
+    Gp = core.psr_batched(*H, 4, shift=E, dscale=-1.0, oscale=-1.0)
+    Gr = core.rgf_batched(*H, shift=E, dscale=-1.0, oscale=-1.0)
+    for k in range(len(E)):
+        p_ = tuple(x[k].cpu().numpy() for x in Gp)
+        r_ = tuple(x[k].cpu().numpy() for x in Gr)
+        assert dense.rel_err(p_, r_) < 1e-11
+
+
+def test_psr_rejects_too_many_partitions(built_lib):
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    d, u, l = configs.dominant_blocks(8, 4, seed=0)
+    with pytest.raises(ValueError):
+        core.psr_batched(d, u, l, 3)
