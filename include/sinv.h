@@ -179,6 +179,10 @@ int sinv_bcr_batched(const sinv_bcr_desc* d, void* work, size_t work_bytes, void
  * partitions, the 2(npart-1)-block reduced system (:313-383, :470-491)
  * solved by RGF.  Partitioning as psr_utils.divide_matrix (:61-99); needs
  * npart >= 2 and nblocks >= 3 npart (psr_utils.check_input :44-55).
+ * With npart == 2 and first_rows = N/2 this is RGF-2sided
+ * (src/sinv/algorithms/rgf/rgf2sided.py:18-274): the two partitions' sweeps
+ * are its half-sweeps and the 2-block reduced system its connection step; it
+ * then needs only 1 <= first_rows < nblocks.
  * Instance convention as sinv_rgf_desc (uniform block size).  Outputs the
  * diagonal, upper and lower blocks of G.
  * ------------------------------------------------------------------------ */
@@ -187,6 +191,7 @@ typedef struct sinv_psr_desc {
   int blocksize;
   int batch;
   int npart;
+  int first_rows;  /* 0: divide_matrix's split; else rows of partition 0 (npart == 2 only) */
   const void* diag;  long long diag_stride;
   const void* upper; long long upper_stride;
   const void* lower; long long lower_stride;

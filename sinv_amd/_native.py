@@ -65,7 +65,7 @@ class BcrDesc(ctypes.Structure):
 
 class PsrDesc(ctypes.Structure):
     _fields_ = [
-        ("nblocks", _i), ("blocksize", _i), ("batch", _i), ("npart", _i),
+        ("nblocks", _i), ("blocksize", _i), ("batch", _i), ("npart", _i), ("first_rows", _i),
         ("diag", _vp), ("diag_stride", _ll),
         ("upper", _vp), ("upper_stride", _ll),
         ("lower", _vp), ("lower_stride", _ll),

@@ -412,7 +412,8 @@ def rgf_adjoint_ragged(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *,
             _unpack(Al, [(bs[i + 1], bs[i]) for i in range(N - 1)]))
 
 
-def psr_batched(diag, upper, lower, npart, *, shift=None, dscale=1.0, oscale=1.0, check=True):
+def psr_batched(diag, upper, lower, npart, *, shift=None, dscale=1.0, oscale=1.0, check=True,
+                first_rows=0):
     """PSR selected inversion (psr_seqsolve.py:20-72) with ``npart`` partitions
     on one GPU: partition-local Schur reduction and production batched over
     partitions, the 2(npart-1)-block reduced system solved by RGF.  Needs
@@ -420,7 +421,10 @@ def psr_batched(diag, upper, lower, npart, *, shift=None, dscale=1.0, oscale=1.0
     outputs as ``rgf_batched``."""
     d, Bd, sd = _batched(to_dev(diag), 3)
     N, b = d.shape[-3], d.shape[-1]
-    if N < 3 * npart or npart < 2:
+    if first_rows:
+        if npart != 2 or not 1 <= first_rows < N:
+            raise ValueError("an explicit first partition needs npart == 2 and 1 <= first_rows < N")
+    elif N < 3 * npart or npart < 2:
         raise ValueError("The number of blocks is to low. There should be at least 3 blockrows "
                          "per partition (and at least 2 partitions)")
     u, Bu, su = _batched(to_dev(upper), 3)
@@ -433,7 +437,8 @@ def psr_batched(diag, upper, lower, npart, *, shift=None, dscale=1.0, oscale=1.0
     Gl = torch.empty((B, N - 1, b, b), dtype=_c128, device=dev)
     info = torch.empty(1, dtype=torch.int64, device=dev)
     desc = nat.PsrDesc(
-        nblocks=N, blocksize=b, batch=B, npart=int(npart), diag=_p(d), diag_stride=sd,
+        nblocks=N, blocksize=b, batch=B, npart=int(npart), first_rows=int(first_rows),
+        diag=_p(d), diag_stride=sd,
         upper=_p(u), upper_stride=su, lower=_p(l), lower_stride=sl, dscale=nat.C128.of(dscale),
         oscale=nat.C128.of(oscale), shift=_p(sh), g_diag=_p(Gd), g_upper=_p(Gu), g_lower=_p(Gl),
         info=_info_ptr(info))

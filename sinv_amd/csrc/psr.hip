@@ -38,7 +38,13 @@ __global__ void merge_info_kernel(unsigned long long* dst, const unsigned long l
 
 bool psr_desc_ok(const sinv_psr_desc* d) {
   if (d == nullptr || d->nblocks <= 0 || d->blocksize <= 0 || d->batch <= 0) return false;
-  if (d->npart < 2 || d->nblocks < 3 * d->npart) return false;
+  if (d->npart < 2) return false;
+  if (d->first_rows != 0) {
+    // explicit split: two partitions of at least one row each (RGF-2sided)
+    if (d->npart != 2 || d->first_rows < 1 || d->first_rows >= d->nblocks) return false;
+  } else if (d->nblocks < 3 * d->npart) {
+    return false;
+  }
   if (d->diag == nullptr || d->upper == nullptr || d->lower == nullptr) return false;
   if (d->g_diag == nullptr || d->g_upper == nullptr || d->g_lower == nullptr) return false;
   if (d->diag_stride < 0 || d->upper_stride < 0 || d->lower_stride < 0) return false;
@@ -57,11 +63,11 @@ struct PsrPlan {
   size_t bytes = 0;
 };
 
-PsrPlan psr_plan(int N, int b, int P, int batch) {
+PsrPlan psr_plan(int N, int b, int P, int batch, int first_rows) {
   PsrPlan Q;
   Q.N = N; Q.b = b; Q.P = P;
   Q.m = N / P;
-  Q.n0 = N - Q.m * (P - 1);
+  Q.n0 = first_rows > 0 ? first_rows : N - Q.m * (P - 1);
   Q.red.push_back(Q.n0 - 1);
   for (int p = 1; p <= P - 2; ++p) {
     const int s = Q.n0 + (p - 1) * Q.m;
@@ -97,17 +103,18 @@ PsrPlan psr_plan(int N, int b, int P, int batch) {
 
 extern "C" size_t sinv_psr_workspace_bytes(const sinv_psr_desc* d) {
   if (!psr_desc_ok(d)) return 0;
-  return psr_plan(d->nblocks, d->blocksize, d->npart, d->batch).bytes;
+  return psr_plan(d->nblocks, d->blocksize, d->npart, d->batch, d->first_rows).bytes;
 }
 
 extern "C" int sinv_psr_batched(const sinv_psr_desc* d, void* work, size_t work_bytes,
                                 void* stream) {
   if (!psr_desc_ok(d)) {
-    set_error("sinv_psr_batched: invalid descriptor (need npart >= 2 and nblocks >= 3 npart)");
+    set_error("sinv_psr_batched: invalid descriptor (need npart >= 2 and nblocks >= 3 npart, "
+              "or npart == 2 with 1 <= first_rows < nblocks)");
     return SINV_E_INVAL;
   }
   const int N = d->nblocks, b = d->blocksize, nb = d->batch, P = d->npart;
-  const PsrPlan Q = psr_plan(N, b, P, nb);
+  const PsrPlan Q = psr_plan(N, b, P, nb, d->first_rows);
   if (work == nullptr || work_bytes < Q.bytes) {
     set_error("sinv_psr_batched: workspace too small (%zu < %zu)", work_bytes, Q.bytes);
     return SINV_E_INVAL;

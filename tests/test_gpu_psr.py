@@ -64,3 +64,38 @@ def test_psr_rejects_too_many_partitions(built_lib):
     d, u, l = configs.dominant_blocks(8, 4, seed=0)
     with pytest.raises(ValueError):
         core.psr_batched(d, u, l, 3)
+
+
+RGF2S_SHAPES = [(4, 1), (8, 2), (12, 3), (128, 8), (128, 16), (128, 32)]
+
+
+@pytest.mark.parametrize("n,b", RGF2S_SHAPES)
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("sym", [False, True])
+@pytest.mark.parametrize("save", [True, False])
+def test_rgf2sided_dropin(built_lib, n, b, cplx, sym, save):
+    from oracle import rgf2s_ref
+    from sinv_amd.algorithms import rgf2s
+    print(f"rgf2sided n={n} b={b} cplx={cplx} sym={sym} save={save}", flush=True)
+    A = mu.generateBandedDiagonalMatrix(n, np.ceil(b / 2), cplx, sym, SEED)
+    d, u, l = mu.convertDenseToBlkTridiag(A, b)
+    got = rgf2s.rgf2sided(d, u, l, sym_mat=sym, save_off_diag=save)
+    ref = rgf2s_ref.rgf2sided_blocks(d, u, l, sym_mat=sym, save_off_diag=save)
+    assert all(g.dtype == d.dtype for g in got)
+    cond = np.linalg.cond(A)
+    assert dense.rel_err(got, ref) < 1e-15 * cond * 10
+    if save:
+        R = mu.convertDenseToBlkTridiag(np.linalg.inv(A), b)
+        for g, r in zip(got, R):
+            assert np.allclose(g, r)
+
+
+@pytest.mark.parametrize("n,b", [(9, 1), (24, 2), (144, 12), (256, 8)])
+def test_psr_seqsolve_dropin(built_lib, n, b):
+    from sinv_amd.algorithms import psr_s
+    A = mu.generateBandedDiagonalMatrix(n, np.ceil(b / 2), True, False, SEED)
+    G = psr_s.psr_seqsolve(A, b)
+    got = mu.convertDenseToBlkTridiag(G, b)
+    ref = mu.convertDenseToBlkTridiag(np.linalg.inv(A), b)
+    for g, r in zip(got, ref):
+        assert np.allclose(g, r)

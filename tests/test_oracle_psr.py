@@ -43,3 +43,35 @@ def test_psr_oracle_rejects_small_partitions():
     d, u, l = configs.dominant_blocks(8, 2, seed=0)
     with pytest.raises(ValueError):
         psr_ref.psr_blocks(d, u, l, 3)
+
+
+# --- RGF-2sided restatement (oracle/rgf2s_ref.py)
+RGF2S_SHAPES = [(4, 1), (8, 2), (12, 3), (128, 8), (128, 16), (128, 32)]
+
+
+@pytest.mark.parametrize("n,b", RGF2S_SHAPES)
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("sym", [False, True])
+def test_rgf2s_oracle_vs_dense(n, b, cplx, sym):
+    """rgf2sided_test.py:36-80: all three bands against the dense inverse."""
+    from oracle import rgf2s_ref
+    A = mu.generateBandedDiagonalMatrix(n, np.ceil(b / 2), cplx, sym, 63)
+    d, u, l = mu.convertDenseToBlkTridiag(A, b)
+    G = rgf2s_ref.rgf2sided_blocks(d, u, l, sym_mat=sym)
+    R = mu.convertDenseToBlkTridiag(np.linalg.inv(A), b)
+    for g, r in zip(G, R):
+        assert np.allclose(g, r)
+
+
+def test_rgf2s_oracle_unsaved_off_diagonals_are_zero_but_the_middle():
+    from oracle import rgf2s_ref
+    from sinv_amd.utils import configs
+    d, u, l = configs.dominant_blocks(10, 3, seed=2)
+    Gd, Gu, Gl = rgf2s_ref.rgf2sided_blocks(d, u, l, save_off_diag=False)
+    R = dense.selected_inverse_dense(d, u, l)
+    assert np.allclose(Gd, R[0])
+    for i in range(9):
+        if i == 4:
+            assert np.allclose(Gu[i], R[1][i]) and np.allclose(Gl[i], R[2][i])
+        else:
+            assert not Gu[i].any() and not Gl[i].any()
