@@ -47,9 +47,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--energies", type=int, default=256)
-    ap.add_argument("--workload", default="energies", choices=["energies", "dominant", "large"])
+    ap.add_argument("--workload", default="energies",
+                    choices=["energies", "dominant", "large", "adjoint"])
+    ap.add_argument("--chunk", type=int, default=64, help="energies per adjoint launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the batch=1 FP64 sample")
+    ap.add_argument("--traffic-pass", action="store_true",
+                    help="run one FP64 step and a 2 GiB calibration copy (neg kernel), untimed, "
+                         "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs; prints no bench line")
     return ap.parse_args()
 
 
@@ -78,6 +83,8 @@ def main():
 
     dev = core.device()
     torch.cuda.synchronize()
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
 
     # ------------------------------------------------------------ workload
     eta = 1e-3
@@ -119,6 +126,23 @@ def main():
         return plans
 
     stream = torch.cuda.current_stream()
+
+    if args.traffic_pass:
+        plans = make_plans("fp64")
+        for p in plans:
+            p.run()
+        torch.cuda.synchronize()
+        for p in plans:
+            p.check()
+        del plans
+        n = 1 << 28  # float64 elements: 2 GiB read + 2 GiB written by one elementwise kernel
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(traffic_pass=wl["workload"], calib_kernel="neg", calib_read_bytes=8 * n,
+                              calib_write_bytes=8 * n)), flush=True)
+        return 0
 
     def timed(plans, steps, warmup, tag):
         for _ in range(warmup):
@@ -298,6 +322,116 @@ def main():
                                 if v[0] > 0 else 0.0) for k, v in results[head]["prof"].items() if v[1]},
     }
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_adjoint(args):
+    """Batched forward + adjoint with checkpointing: 512 energies in [-2, 2],
+    eta = 1e-3, one Hermitian H of 96 blocks whose sizes mix 256 and 416
+    inside the instance; loss L = -(1/pi) sum_E sum_i Im tr G_ii(E); the
+    gradient dL/dH on the block-tridiagonal pattern, summed over energies.
+    A step is the loss and its full gradient over all energies, processed in
+    energy chunks (left-connected blocks checkpointed every 8 blocks)."""
+    import torch
+    from sinv_amd import core, _native
+    from sinv_amd.utils import configs
+
+    rng = np.random.default_rng(31)
+    bs = [int(x) for x in rng.choice([256, 416], size=96)]
+    H = configs.hermitian_H_ragged(bs, seed=32)
+    nE = args.energies if args.energies != 256 else 512
+    E = configs.energies(nE, seed=33) + 1e-3j
+    Hd = [core.to_dev(x) for x in H[0]]
+    Hu = [core.to_dev(x) for x in H[1]]
+    Hl = [core.to_dev(x) for x in H[2]]
+    gd = [(-1j / np.pi) * torch.eye(b, dtype=torch.complex128, device=Hd[0].device) for b in bs]
+    zu = [torch.zeros_like(x) for x in Hu]
+    zl = [torch.zeros_like(x) for x in Hl]
+    every = 8
+
+    def step(prec, limit=None):
+        loss = torch.zeros((), dtype=torch.float64, device=Hd[0].device)
+        grad = [torch.zeros_like(x) for x in Hd + Hu + Hl]
+        for s0 in range(0, nE if limit is None else limit, args.chunk):
+            sh = E[s0:s0 + args.chunk]
+            # one adjoint call gives the gradient and, from its sweep over A^H, G_ii for the loss
+            Ad, Au, Al, Gd = core.rgf_adjoint_ragged(Hd, Hu, Hl, gd, zu, zl, shift=sh, dscale=-1.0,
+                                                     oscale=-1.0, precision=prec,
+                                                     checkpoint_every=every, check=False,
+                                                     return_g_diag=True)
+            for g in Gd:
+                loss -= torch.diagonal(g, dim1=-2, dim2=-1).imag.sum() / np.pi
+            for acc, a in zip(grad, Ad + Au + Al):
+                acc -= a.sum(0)          # A = (E + i eta) I - H: dL/dH = -sum_E dL/dA
+            del Gd, Ad, Au, Al
+        return loss, grad
+
+    out = {}
+    for prec in ("fp64", "mixed"):
+        log(f"[bench] adjoint workload {prec}: warmup")
+        torch.cuda.reset_peak_memory_stats()
+        for _ in range(args.warmup):
+            step(prec)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            loss, grad = step(prec)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        peak = torch.cuda.max_memory_allocated() / 1e9
+        log(f"[bench] {prec}: {ms:.1f} ms/step, {nE / (ms / 1e3):.1f} solves/s (fwd+adj), "
+            f"peak {peak:.1f} GB")
+        # kernel-level figures from one energy chunk (same shapes as every chunk)
+        with _native.LaunchProfiler(max_launches=100000) as prof:
+            step(prec, limit=args.chunk)
+            torch.cuda.synchronize()
+        out[prec] = dict(ms=ms, loss=float(loss), grad=grad, prof=prof.stats, peak_gb=peak)
+    num = sum(float(((a - b).abs() ** 2).sum()) for a, b in zip(out["mixed"]["grad"], out["fp64"]["grad"]))
+    den = sum(float((b.abs() ** 2).sum()) for b in out["fp64"]["grad"])
+    gerr = float(np.sqrt(num / den))
+    lerr = abs(out["mixed"]["loss"] - out["fp64"]["loss"]) / abs(out["fp64"]["loss"])
+    fwd_flops = configs.rgf_flops_ragged(bs) * nE
+    head = "fp64"
+    ms = out[head]["ms"]
+    st = out[head]["prof"]
+    # the profiled chunk is one of ceil(nE / chunk) identical chunks
+    nchunks = -(-nE // args.chunk)
+    executed = sum(v[2] for v in st.values()) * nchunks
+    kms, kcnt, kfl = st["gemm_f64"]
+    achieved = kfl / (kms / 1e3) / 1e12 if kms > 0 else 0.0
+    res = {
+        "metric": METRIC,
+        "value": round(nE / (ms / 1e3), 3),
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
+        "config": dict(workload="rgf_forward_adjoint_dos", blocks=96, block_sizes="256/416 mixed",
+                       energies=nE, eta=1e-3, checkpoint_every=every, energy_chunk=args.chunk,
+                       solve="DOS loss and dL/dH per energy from one adjoint sweep (G_ii "
+                             "and the gradient come out of the same sweep over A^H)"),
+        "executed_tflops": round(executed / (ms / 1e3) / 1e12, 3),
+        "forward_algorithmic_tflops": round(fwd_flops / (ms / 1e3) / 1e12, 3),
+        "peak_hbm_gb": round(out[head]["peak_gb"], 2),
+        "modes": {
+            "fp64": dict(solves_per_s=round(nE / (out["fp64"]["ms"] / 1e3), 3),
+                         ms_per_step=round(out["fp64"]["ms"], 3), peak_hbm_gb=round(out["fp64"]["peak_gb"], 2)),
+            "mixed": dict(solves_per_s=round(nE / (out["mixed"]["ms"] / 1e3), 3),
+                          ms_per_step=round(out["mixed"]["ms"], 3),
+                          grad_rel_err_vs_fp64=gerr, loss_rel_err_vs_fp64=lerr,
+                          passes_err_gate=gerr <= ERR_GATE),
+        },
+        "roofline": dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_TFLOPS["gemm_f64"],
+                         unit="TFLOP/s", frac=round(achieved / PEAK_TFLOPS["gemm_f64"], 4),
+                         traffic=None, kernel="gemm_kernel<double>", launches=kcnt,
+                         avg_launch_us=round(kms / max(kcnt, 1) * 1e3, 3)),
+        "cpu_baseline": None,
+        "kernel_time": {k: dict(ms=round(v[0], 3), launches=v[1]) for k, v in st.items() if v[1]},
+    }
+    print(json.dumps(res), flush=True)
     return 0
 
 

@@ -109,6 +109,9 @@ int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_bytes, void
  * checkpoint_every = k > 0 stores the left-connected blocks only every k
  * blocks and recomputes each k-block segment during the backward sweep
  * (0 = store all).  gbar_* strides may be 0 (one grad_G for every instance).
+ * The sweep over A^H passes through G_ii^H, so a non-NULL g_diag receives
+ * the selected inverse's diagonal blocks as well (a loss and its gradient
+ * from one call, without a separate forward solve).
  * ------------------------------------------------------------------------ */
 typedef struct sinv_rgf_adjoint_desc {
   int nblocks;
@@ -129,6 +132,7 @@ typedef struct sinv_rgf_adjoint_desc {
   void* abar_diag;
   void* abar_upper;
   void* abar_lower;
+  void* g_diag;  /* optional: also the diagonal blocks G_ii of the forward solve */
   unsigned long long* info;
 } sinv_rgf_adjoint_desc;
 

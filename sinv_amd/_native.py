@@ -90,6 +90,7 @@ class AdjDesc(ctypes.Structure):
         ("gbar_upper", _vp), ("gbar_upper_stride", _ll),
         ("gbar_lower", _vp), ("gbar_lower_stride", _ll),
         ("abar_diag", _vp), ("abar_upper", _vp), ("abar_lower", _vp),
+        ("g_diag", _vp),
         ("info", ctypes.POINTER(ctypes.c_ulonglong)),
     ]
 

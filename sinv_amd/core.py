@@ -235,7 +235,7 @@ def bcr_batched(diag, upper, lower, *, shift=None, dscale=1.0, oscale=1.0, retur
 
 def rgf_adjoint_batched(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *, shift=None,
                         dscale=1.0, oscale=1.0, precision="fp64", checkpoint_every=0,
-                        check=True):
+                        check=True, return_g_diag=False):
     """Gradient of a real loss w.r.t. the blocks of A, given its gradient w.r.t.
     the selected blocks of G = A^-1: grad_A = -G^H grad_G G^H on the pattern.
 
@@ -265,6 +265,7 @@ def rgf_adjoint_batched(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *
     Ad = torch.empty((B, N, b, b), dtype=_c128, device=dev)
     Au = torch.empty((B, max(N - 1, 0), b, b), dtype=_c128, device=dev)
     Al = torch.empty((B, max(N - 1, 0), b, b), dtype=_c128, device=dev)
+    Gd = torch.empty((B, N, b, b), dtype=_c128, device=dev) if return_g_diag else None
     info = torch.empty(1, dtype=torch.int64, device=dev)
     desc = nat.AdjDesc(
         nblocks=N, blocksize=b, batch=B, precision=nat.PREC[precision],
@@ -273,13 +274,16 @@ def rgf_adjoint_batched(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *
         dscale=nat.C128.of(dscale), oscale=nat.C128.of(oscale), shift=_p(sh),
         gbar_diag=_p(gd), gbar_diag_stride=sgd, gbar_upper=_p(gu), gbar_upper_stride=sgu,
         gbar_lower=_p(gl), gbar_lower_stride=sgl,
-        abar_diag=_p(Ad), abar_upper=_p(Au), abar_lower=_p(Al), info=_info_ptr(info))
+        abar_diag=_p(Ad), abar_upper=_p(Au), abar_lower=_p(Al), g_diag=_p(Gd),
+        info=_info_ptr(info))
     L = nat.lib()
     ws = _workspace(L.sinv_rgf_adjoint_workspace_bytes(ctypes.byref(desc)))
     nat.check(L.sinv_rgf_adjoint_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()),
               "rgf_adjoint")
     if check:
         nat.check_info(info)
+    if return_g_diag:
+        return Ad, Au, Al, Gd
     return Ad, Au, Al
 
 
@@ -369,7 +373,8 @@ def rgf_ragged(diag, upper, lower, *, sym_mat=False, save_off_diag=True, precisi
 
 
 def rgf_adjoint_ragged(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *, shift=None,
-                       dscale=1.0, oscale=1.0, precision="fp64", checkpoint_every=0, check=True):
+                       dscale=1.0, oscale=1.0, precision="fp64", checkpoint_every=0, check=True,
+                       return_g_diag=False):
     """``rgf_adjoint_batched`` with per-block sizes (lists of blocks, as
     ``rgf_ragged``).  Returns lists of gradient blocks (B, ...)."""
     if precision not in nat.PREC:
@@ -391,6 +396,7 @@ def rgf_adjoint_ragged(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *,
     Ad = torch.empty((B, dsize), dtype=_c128, device=dev)
     Au = torch.empty((B, max(osize, 1)), dtype=_c128, device=dev)
     Al = torch.empty((B, max(osize, 1)), dtype=_c128, device=dev)
+    Gd = torch.empty((B, dsize), dtype=_c128, device=dev) if return_g_diag else None
     info = torch.empty(1, dtype=torch.int64, device=dev)
     bsz = (ctypes.c_int * N)(*bs)
     desc = nat.AdjDesc(
@@ -400,16 +406,20 @@ def rgf_adjoint_ragged(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *,
         dscale=nat.C128.of(dscale), oscale=nat.C128.of(oscale), shift=_p(sh),
         gbar_diag=_p(gd), gbar_diag_stride=sgd, gbar_upper=_p(gu), gbar_upper_stride=sgu,
         gbar_lower=_p(gl), gbar_lower_stride=sgl,
-        abar_diag=_p(Ad), abar_upper=_p(Au), abar_lower=_p(Al), info=_info_ptr(info))
+        abar_diag=_p(Ad), abar_upper=_p(Au), abar_lower=_p(Al), g_diag=_p(Gd),
+        info=_info_ptr(info))
     L = nat.lib()
     ws = _workspace(L.sinv_rgf_adjoint_workspace_bytes(ctypes.byref(desc)))
     nat.check(L.sinv_rgf_adjoint_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()),
               "rgf_adjoint")
     if check:
         nat.check_info(info)
-    return (_unpack(Ad, [(b, b) for b in bs]),
-            _unpack(Au, [(bs[i], bs[i + 1]) for i in range(N - 1)]),
-            _unpack(Al, [(bs[i + 1], bs[i]) for i in range(N - 1)]))
+    out = (_unpack(Ad, [(b, b) for b in bs]),
+           _unpack(Au, [(bs[i], bs[i + 1]) for i in range(N - 1)]),
+           _unpack(Al, [(bs[i + 1], bs[i]) for i in range(N - 1)]))
+    if return_g_diag:
+        return out + (_unpack(Gd, [(b, b) for b in bs]),)
+    return out
 
 
 def psr_batched(diag, upper, lower, npart, *, shift=None, dscale=1.0, oscale=1.0, check=True,

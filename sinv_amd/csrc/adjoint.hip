@@ -221,6 +221,10 @@ extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* wo
   const int bl = bs(N - 1);
   SINV_HIP_TRY(copy(bl, bl, slot_g(N - 1), WI, Gpp[0], WI));                  // G_{N-1,N-1}(B)
   SINV_HIP_TRY(copy(bl, bl, slot_dg(N - 1), WI, Ad + Ly.doff[N - 1], AdI));  // its tangent
+  cplx* Gout = (cplx*)d->g_diag;  // optional G_ii = (G_ii of B)^H
+  if (Gout != nullptr)
+    SINV_HIP_TRY(launch_transpose(bl, bl, OP_C, Gpp[0], bl, 0, WI, Gout + Ly.doff[N - 1], bl, 0,
+                                  AdI, 1, nb, s));
   for (int seg = P.nseg - 1; seg >= 0; --seg) {
     const int i0 = seg * P.k, i1 = (seg + 1) * P.k < N ? (seg + 1) * P.k : N;
     if (seg != P.nseg - 1) {
@@ -258,6 +262,9 @@ extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* wo
       // tangent of G_ii: dg - S Xn - Q dXn
       SINV_HIP_TRY(gemm(bi, bi, bj, ws(Sb, bj), ws(Xn, bi), Adi, AdI, bi, mone, dg, WI));
       SINV_HIP_TRY(mm(bi, bi, bj, ws(Qb, bj), Opd{Ali, AoI, OP_N, bi}, Adi, AdI, bi, mone, true));
+      if (Gout != nullptr)
+        SINV_HIP_TRY(launch_transpose(bi, bi, OP_C, Gi, bi, 0, WI, Gout + Ly.doff[i], bi, 0, AdI,
+                                      1, nb, s));
       cur ^= 1;
     }
   }

@@ -1,6 +1,13 @@
 """Summaries of rocprofv3 output for the hot-path kernels.
 
     python -m sinv_amd.profiling <rocprofv3 output dir> [--range NAME]
+    python -m sinv_amd.profiling --traffic FETCH_DIR WRITE_DIR --calib-bytes BYTES
+
+The second form reads two ``rocprofv3 --pmc`` runs (``FETCH_SIZE``, then
+``WRITE_SIZE``, both in KiB per dispatch) of ``bench.py --traffic-pass`` and
+reports the mean HBM bytes per dispatch of each hot-path kernel, each counter
+scaled by the factor that makes the calibration kernel (one elementwise
+``neg`` over BYTES read and BYTES written) read exactly its known size.
 
 Reads the ``*_kernel_stats.csv`` (per-kernel totals over the whole run) and,
 when present, ``*_kernel_trace.csv`` plus ``*_marker_api_trace.csv`` to give
@@ -74,7 +81,58 @@ def range_table(d, name):
     return [(k, n, t / 1e3, t / n) for k, (n, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])]
 
 
+def pmc_means(d, counter):
+    """Mean value of `counter` per dispatch, by kernel (summed over the rows of one dispatch)."""
+    f = (_one(os.path.join(d, "**", "*counter_collection.csv"))
+         or _one(os.path.join(d, "*counter_collection.csv")))
+    if not f:
+        return {}
+    per = defaultdict(float)
+    name = {}
+    for r in csv.DictReader(open(f)):
+        if r.get("Counter_Name") != counter:
+            continue
+        k = r.get("Dispatch_Id") or r.get("Correlation_Id")
+        per[k] += float(r["Counter_Value"])
+        name[k] = r["Kernel_Name"]
+    agg = defaultdict(lambda: [0, 0.0])
+    for k, v in per.items():
+        a = agg[_short(name[k])]
+        a[0] += 1
+        a[1] += v
+    return {k: (n, t / n) for k, (n, t) in agg.items()}
+
+
+def traffic_table(fetch_dir, write_dir, calib_bytes):
+    fetch = pmc_means(fetch_dir, "FETCH_SIZE")
+    write = pmc_means(write_dir, "WRITE_SIZE")
+    cal = [k for k in fetch if "neg" in k]
+    if not cal or cal[0] not in write:
+        raise SystemExit("calibration kernel (neg) not found in both runs")
+    kf = calib_bytes / (fetch[cal[0]][1] * 1024.0)
+    kw = calib_bytes / (write[cal[0]][1] * 1024.0)
+    rows = []
+    for k in sorted(fetch, key=lambda x: -fetch[x][0]):
+        if k in write:
+            n, f = fetch[k]
+            rows.append((k, n, f * 1024.0 * kf, write[k][1] * 1024.0 * kw))
+    return kf, kw, rows
+
+
 def main(argv=None):
+    argv = sys.argv[1:] if argv is None else argv
+    if argv and argv[0] == "--traffic":
+        ap = argparse.ArgumentParser()
+        ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
+        ap.add_argument("--calib-bytes", type=float, required=True)
+        a = ap.parse_args(argv)
+        kf, kw, rows = traffic_table(a.traffic[0], a.traffic[1], a.calib_bytes)
+        print(f"# HBM traffic per dispatch from FETCH_SIZE ({a.traffic[0]}) and WRITE_SIZE ({a.traffic[1]})")
+        print(f"calibration (neg over {a.calib_bytes:.0f} B): fetch factor {kf:.4f}, write factor {kw:.4f}")
+        print(f"{'kernel':28s} {'dispatches':>10s} {'fetch_MB':>10s} {'write_MB':>10s} {'total_MB':>10s}")
+        for k, n, f, w in rows:
+            print(f"{k:28s} {n:10d} {f / 1e6:10.3f} {w / 1e6:10.3f} {(f + w) / 1e6:10.3f}")
+        return 0
     ap = argparse.ArgumentParser()
     ap.add_argument("dir")
     ap.add_argument("--range", action="append", default=[])

@@ -117,3 +117,18 @@ def test_dos_gradient_32x128(built_lib):
             i, j = rng.integers(0, b, size=2)
             fd = adjoint_ref.fd_grad(loss, *H, which, (blk, i, j), h=1e-6)
             assert abs(fd - grad[blk, i, j]) < 1e-5 * max(1.0, abs(grad[blk, i, j])), (which, fd, grad[blk, i, j])
+
+
+def test_adjoint_returns_forward_diagonal(built_lib):
+    """The adjoint's sweep over A^H yields G_ii as well: equal to the forward RGF."""
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    H = configs.hermitian_H(10, 48, seed=21)
+    E = configs.energies(3, seed=22) + 1e-2j
+    gd = adjoint_ref.dos_grad_G((10, 48))
+    z = np.zeros((9, 48, 48), complex)
+    Ad, Au, Al, Gd = core.rgf_adjoint_batched(*H, gd, z, z, shift=E, dscale=-1, oscale=-1,
+                                              checkpoint_every=3, return_g_diag=True)
+    Gf = core.rgf_batched(*H, shift=E, dscale=-1, oscale=-1)[0]
+    err = float(((Gd - Gf).abs() ** 2).sum().sqrt() / (Gf.abs() ** 2).sum().sqrt())
+    assert err < 1e-12
