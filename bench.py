@@ -58,6 +58,21 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per dispatch of `kernel` from the committed PMC summary of this
+    workload (FETCH_SIZE + WRITE_SIZE per dispatch, calibrated on a copy kernel),
+    or None when no summary is committed."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 5 and f[0] == kernel:
+            return float(f[4]) * 1e6
+    return None
+
+
 def rel_err_per_instance(G, R):
     """max over instances of ||sel(G)-sel(R)||_F / ||sel(R)||_F (torch tensors, dim 0 = batch)."""
     import torch
@@ -177,7 +192,7 @@ def main():
             for p in plans:
                 p.run()
             torch.cuda.synchronize()
-        results[prec] = dict(ms=ms, prof=prof.stats)
+        results[prec] = dict(ms=ms, prof=prof.stats, bytes=prof.bytes)
         del plans
         torch.cuda.synchronize()
 
@@ -286,12 +301,16 @@ def main():
     ms = results[head]["ms"]
     kind = "gemm_f64" if head == "fp64" else "gemm_f32"
     kms, kcnt, kfl = results[head]["prof"][kind]
+    kname = f"gemm_kernel<{'double' if head == 'fp64' else 'float'}>"
     achieved = kfl / (kms / 1e3) / 1e12 if kms > 0 else 0.0
     roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_TFLOPS[kind], unit="TFLOP/s",
-                frac=round(achieved / PEAK_TFLOPS[kind], 4), traffic=None,
-                kernel=f"gemm_kernel<{'double' if head == 'fp64' else 'float'}>",
-                launches=kcnt, avg_launch_us=round(kms / max(kcnt, 1) * 1e3, 3),
-                flops_per_launch=kfl / max(kcnt, 1))
+                frac=round(achieved / PEAK_TFLOPS[kind], 4),
+                traffic=measured_traffic(wl["workload"], kname),
+                kernel=kname, launches=kcnt, avg_launch_us=round(kms / max(kcnt, 1) * 1e3, 3),
+                flops_per_launch=kfl / max(kcnt, 1),
+                algorithmic_bytes_per_launch=results[head]["bytes"][kind] / max(kcnt, 1),
+                traffic_source=f"profiles/traffic_{wl['workload']}.txt (rocprofv3 --pmc "
+                               "FETCH_SIZE + WRITE_SIZE, calibrated; null if absent)")
     out = {
         "metric": METRIC,
         "value": round(solves / (ms / 1e3), 3),

@@ -244,12 +244,15 @@ int sinv_zgemm_batched(int M, int N, int K, int opA, int opB, sinv_c128 alpha,
  * and leaf-inverse kernels is bracketed by hipEvents on its own stream.
  * sinv_prof_end synchronises those events and returns, per kind
  * (0 = FP64 GEMM, 1 = FP32 GEMM, 2 = leaf inverse, 3 = other), the summed
- * kernel time in ms, the launch count and the algorithmic flops
- * (8 M N K per complex GEMM, 8 n^3 per inverse).  Process-wide state: use
- * from one thread, around a region with at most `max_launches` launches.
+ * kernel time in ms, the launch count, the algorithmic flops (8 M N K per
+ * complex GEMM, 8 n^3 per inverse) and, when `bytes` is not NULL, the
+ * algorithmic bytes (16 B per complex element of each operand read and each
+ * result written once: M K + K N + M N, + M N more with a C0 term; 2 n^2 per
+ * in-place inverse).  Process-wide state: use from one thread, around a
+ * region with at most `max_launches` launches.
  * ------------------------------------------------------------------------ */
 int sinv_prof_begin(int max_launches);
-int sinv_prof_end(double* ms, long long* launches, double* flops, int nkinds);
+int sinv_prof_end(double* ms, long long* launches, double* flops, double* bytes, int nkinds);
 
 #ifdef __cplusplus
 }

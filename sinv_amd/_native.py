@@ -126,7 +126,8 @@ def lib():
     L.sinv_prof_begin.argtypes = [_i]
     L.sinv_prof_end.restype = _i
     L.sinv_prof_end.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_ll),
-                                ctypes.POINTER(ctypes.c_double), _i]
+                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                _i]
     if hasattr(L, "sinv_bcr_batched"):
         L.sinv_bcr_workspace_bytes.restype = ctypes.c_size_t
         L.sinv_bcr_workspace_bytes.argtypes = [ctypes.POINTER(BcrDesc)]
@@ -178,7 +179,8 @@ PROF_KINDS = ("gemm_f64", "gemm_f32", "inv_leaf", "other")
 class LaunchProfiler:
     """Context manager around libsinv_hip's launch profiler.
 
-    ``stats`` maps kernel kind -> (total ms, launches, algorithmic flops)."""
+    ``stats`` maps kernel kind -> (total ms, launches, algorithmic flops);
+    ``bytes`` maps kernel kind -> algorithmic bytes."""
 
     def __init__(self, max_launches=200000):
         self.max_launches = max_launches
@@ -193,6 +195,8 @@ class LaunchProfiler:
         ms = (ctypes.c_double * n)()
         cnt = (ctypes.c_longlong * n)()
         fl = (ctypes.c_double * n)()
-        check(lib().sinv_prof_end(ms, cnt, fl, n), "prof_end")
+        by = (ctypes.c_double * n)()
+        check(lib().sinv_prof_end(ms, cnt, fl, by, n), "prof_end")
         self.stats = {k: (ms[i], cnt[i], fl[i]) for i, k in enumerate(PROF_KINDS)}
+        self.bytes = {k: by[i] for i, k in enumerate(PROF_KINDS)}
         return False

@@ -398,8 +398,11 @@ hipError_t launch_gemm(const GemmArgs& g, Prec p, hipStream_t s) {
   else
     hipLaunchKernelGGL(gemm_kernel<float>, dim3((unsigned)blocks), dim3(NTHR), 0, s, g, tiles_m,
                        tiles_n);
+  const double nb = g.inner * (double)g.outer;
+  const double elems = (double)g.M * g.K + (double)g.K * g.N +
+                       (double)g.M * g.N * (g.C0 != nullptr ? 2.0 : 1.0);
   prof_close(pid, s, p == Prec::F64 ? PROF_GEMM_F64 : PROF_GEMM_F32,
-             8.0 * g.M * (double)g.N * g.K * g.inner * (double)g.outer);
+             8.0 * g.M * (double)g.N * g.K * nb, sizeof(cplx) * elems * nb);
   return hipGetLastError();
 }
 

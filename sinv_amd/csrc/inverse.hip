@@ -163,7 +163,8 @@ hipError_t launch_inverse(cplx* A, int n, int lda, long long s_in, long long s_o
     const int pid = prof_open(s);
     hipLaunchKernelGGL(inv_leaf_kernel, dim3((unsigned)(inner * outer)), dim3(256), 0, s, A, n,
                        lda, s_in, s_out, inner, info, tag);
-    prof_close(pid, s, PROF_INV_LEAF, 8.0 * n * (double)n * n * inner * (double)outer);
+    prof_close(pid, s, PROF_INV_LEAF, 8.0 * n * (double)n * n * inner * (double)outer,
+               2.0 * sizeof(cplx) * n * (double)n * inner * (double)outer);
     return hipGetLastError();
   }
   const int n1 = split_point(n), n2 = n - n1;

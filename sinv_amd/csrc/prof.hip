@@ -1,6 +1,6 @@
 // Launch-level profiler: while enabled, each GEMM / leaf-inverse launch is
 // bracketed by a pair of hipEvents recorded on the launch's own stream, and
-// its algorithmic flops are tallied per kernel kind.  Used to measure the
+// its algorithmic flops and bytes are tallied per kernel kind.  Used to measure the
 // dominant kernel's average duration without an external profiler.
 // Process-wide state: enable it around a single-threaded region only.
 #include <vector>
@@ -15,6 +15,7 @@ struct Rec {
   hipEvent_t a, b;
   int kind;
   double flops;
+  double bytes;
 };
 
 struct Prof {
@@ -39,11 +40,12 @@ int prof_open(hipStream_t s) {
   return id;
 }
 
-void prof_close(int id, hipStream_t s, int kind, double flops) {
+void prof_close(int id, hipStream_t s, int kind, double flops, double bytes) {
   if (id < 0) return;
   (void)hipEventRecord(g_prof.recs[id].b, s);
   g_prof.recs[id].kind = kind;
   g_prof.recs[id].flops = flops;
+  g_prof.recs[id].bytes = bytes;
 }
 
 }  // namespace sinv
@@ -61,6 +63,7 @@ extern "C" int sinv_prof_begin(int max_launches) {
     SINV_HIP_TRY(hipEventCreate(&r.b));
     r.kind = -1;
     r.flops = 0.0;
+    r.bytes = 0.0;
   }
   g_prof.next = 0;
   g_prof.overflow = false;
@@ -68,7 +71,8 @@ extern "C" int sinv_prof_begin(int max_launches) {
   return SINV_OK;
 }
 
-extern "C" int sinv_prof_end(double* ms, long long* launches, double* flops, int nkinds) {
+extern "C" int sinv_prof_end(double* ms, long long* launches, double* flops, double* bytes,
+                             int nkinds) {
   if (!g_prof.on) {
     set_error("sinv_prof_end: profiler not active");
     return SINV_E_INVAL;
@@ -78,6 +82,7 @@ extern "C" int sinv_prof_end(double* ms, long long* launches, double* flops, int
     ms[k] = 0.0;
     launches[k] = 0;
     flops[k] = 0.0;
+    if (bytes != nullptr) bytes[k] = 0.0;
   }
   int rc = SINV_OK;
   for (size_t i = 0; i < g_prof.next; ++i) {
@@ -89,6 +94,7 @@ extern "C" int sinv_prof_end(double* ms, long long* launches, double* flops, int
       ms[r.kind] += t;
       launches[r.kind] += 1;
       flops[r.kind] += r.flops;
+      if (bytes != nullptr) bytes[r.kind] += r.bytes;
     }
   }
   for (auto& r : g_prof.recs) {

@@ -113,10 +113,11 @@ inline BlockLayout make_block_layout(int N, int blocksize, const int* bsizes) {
 void set_error(const char* fmt, ...);
 
 // launch profiler (prof.hip): prof_open before a launch returns an id (-1 when
-// off), prof_close after it records the kernel kind and its algorithmic flops.
+// off), prof_close after it records the kernel kind and its algorithmic flops
+// and bytes (the operands read and results written once each, as stored).
 enum ProfKind : int { PROF_GEMM_F64 = 0, PROF_GEMM_F32 = 1, PROF_INV_LEAF = 2, PROF_OTHER = 3 };
 int prof_open(hipStream_t s);
-void prof_close(int id, hipStream_t s, int kind, double flops);
+void prof_close(int id, hipStream_t s, int kind, double flops, double bytes);
 
 }  // namespace sinv
 
