@@ -1,11 +1,11 @@
 // K1: batched in-place complex128 block inverse.
 //
-// Leaves (n <= 64): one workgroup per matrix, the whole block resident in LDS
-// (64 x 65 complex128 = 65 KiB), Gauss-Jordan elimination with partial
-// pivoting.  The pivot is the first row maximising |re| + |im| in the column
-// (the choice LAPACK's izamax makes inside ?getrf, which numpy.linalg.inv
-// calls), found with one wave64 shuffle reduction per step.  Row swaps are
-// undone at the end as a column gather.
+// Leaves (n <= 64): one workgroup per matrix, Gauss-Jordan elimination with
+// partial pivoting.  The pivot is the first row maximising |re| + |im| in the
+// column (the choice LAPACK's izamax makes inside ?getrf, which
+// numpy.linalg.inv calls).  Default kernel: the block lives in registers, one
+// row per lane (inv_leaf_kernel_rows); the older LDS-resident kernel
+// (inv_leaf_kernel) is kept as a selectable alternative.
 //
 // Larger blocks: 2 x 2 Schur-complement recursion, all of whose arithmetic is
 // batched GEMM on FP64 MFMA:
@@ -15,6 +15,9 @@
 // leading sub-blocks, which diagonally dominant blocks and blocks whose
 // anti-Hermitian part is definite (the Schur complements of (E + i eta) I - H,
 // eta > 0) always have.  Cost 8 n^3 real flops either way.
+#include <cstdlib>
+#include <cstring>
+
 #include "sinv_core.hpp"
 
 namespace sinv {
@@ -135,6 +138,208 @@ __global__ __launch_bounds__(256) void inv_leaf_kernel(cplx* A, int n, int lda, 
     }
 }
 
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)v, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// DPP controls (gfx9 encoding) of the wave64 argmax below
+constexpr int kDppXor1 = 0xB1;       // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;       // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141;
+constexpr int kDppMirror = 0x140;
+constexpr int kDppBcast15 = 0x142;
+constexpr int kDppBcast31 = 0x143;
+
+// One combine step of the pivot argmax: candidate (m, key) from the DPP
+// source lane wins if its modulus is larger, or equal with a smaller key
+// (key = position * 64 + row, so ties go to the first row in position order).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void amax_step(double& m, int& key) {
+  const long long mb = __double_as_longlong(m);
+  const int lo = __builtin_amdgcn_update_dpp((int)mb, (int)mb, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(mb >> 32), (int)(mb >> 32), CTRL, ROWS, 0xF,
+                                             false);
+  const int ok = __builtin_amdgcn_update_dpp(key, key, CTRL, ROWS, 0xF, false);
+  const double om = __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+  if (om > m || (om == m && ok < key)) {
+    m = om;
+    key = ok;
+  }
+}
+
+__device__ __forceinline__ void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Register-resident leaf, row layout: thread (w, lane i) keeps row i,
+// columns w*R .. w*R+R-1 (R = 64/W), in registers for the whole elimination.
+// Rows are never moved: each row carries its position in the row order that
+// swapping Gauss-Jordan would have, the pivot search picks the first maximum
+// of |re| + |im| in that order among unused rows (izamax's choice), and the
+// positions are applied when the result is written.  Per step the wave that
+// owns column k finds the pivot with a DPP argmax and publishes column k, the
+// pivot row index/position and 1/pivot (one barrier, double-buffered); each
+// wave then reads its segment of the pivot row from the pivot lane through a
+// wave-private LDS slot and applies  a <- keep * a - g * P  to every element
+// (g = a_ik / pivot, or -1/pivot on the pivot row, whose keep is 0; column
+// k's entry of P is 1 and column k itself is cleared first).  Rows and
+// columns past n are identity padding, untouched by the n steps.
+template <int W>
+__global__ __launch_bounds__(64 * W) void inv_leaf_kernel_rows(cplx* A, int n, int lda,
+                                                               long long s_in, long long s_out,
+                                                               int inner,
+                                                               unsigned long long* info,
+                                                               int tag) {
+  constexpr int R = 64 / W;
+  __shared__ cplx colk[2][64];
+  __shared__ cplx dinv[2];
+  __shared__ int pivrow[2];
+  __shared__ int pivpos[64];
+  __shared__ cplx prw[W][R];
+  __shared__ int perm[64];
+  __shared__ int invp[64];
+
+  const int z = blockIdx.x;
+  const int zo = z / inner, zi = z - zo * inner;
+  A += zi * s_in + zo * s_out;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c0 = w * R;
+
+  cplx a[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int c = c0 + r;
+    if (lane < n && c < n) {
+      const double2 v = *reinterpret_cast<const double2*>(A + (long long)lane * lda + c);
+      a[r] = cplx{v.x, v.y};
+    } else {
+      a[r] = cplx{lane == c ? 1.0 : 0.0, 0.0};
+    }
+  }
+
+  int pos = lane;  // position of this row in the swapped row order
+  bool singular = false;
+  for (int kb = 0; kb < W; ++kb) {
+#pragma unroll
+    for (int kr = 0; kr < R; ++kr) {
+      const int k = kb * R + kr;
+      if (k < n) {
+        const int buf = k & 1;
+        if (w == kb) {
+          // ---- owner of column k: pivot search, publish column k and 1/pivot
+          const cplx f = a[kr];
+          double m = fabs(f.re) + fabs(f.im);
+          if (!(m >= 0.0) || pos < k || lane >= n) m = -1.0;  // used rows, padding, NaN
+          int key = pos * 64 + lane;
+          amax_step<kDppXor1, 0xF>(m, key);
+          amax_step<kDppXor2, 0xF>(m, key);
+          amax_step<kDppHalfMirror, 0xF>(m, key);
+          amax_step<kDppMirror, 0xF>(m, key);
+          amax_step<kDppBcast15, 0xA>(m, key);
+          amax_step<kDppBcast31, 0xC>(m, key);
+          const long long mbits = __double_as_longlong(m);
+          const double mw = __longlong_as_double(
+              ((long long)__builtin_amdgcn_readlane((int)(mbits >> 32), 63) << 32) |
+              (long long)(unsigned)__builtin_amdgcn_readlane((int)mbits, 63));
+          int kw = __builtin_amdgcn_readlane(key, 63);
+          int p, pp;
+          if (mw >= 0.0) {
+            p = kw & 63;
+            pp = kw >> 6;
+          } else {  // no comparable candidate: keep the row at position k
+            const unsigned long long bal = __ballot(pos == k);
+            p = (int)__builtin_ctzll(bal);
+            pp = k;
+          }
+          if (mw == 0.0) singular = true;
+          const cplx pv{readlane_f64(f.re, p), readlane_f64(f.im, p)};
+          const cplx d = crecip(pv);
+          colk[buf][lane] = f;
+          if (lane == 0) {
+            dinv[buf] = d;
+            pivrow[buf] = p;
+            pivpos[k] = pp;
+          }
+        }
+        __syncthreads();
+        // ---- every wave: eliminate column k with the published pivot
+        const int p = __builtin_amdgcn_readfirstlane(pivrow[buf]);
+        const int pp = pivpos[k];
+        const cplx d = dinv[buf];
+        const cplx f = colk[buf][lane];
+        cplx g = cmul(f, d);
+        double keep = 1.0;
+        if (lane == p) {
+          g = cplx{-d.re, -d.im};
+          keep = 0.0;
+        }
+        if (lane == p) {
+  #pragma unroll
+          for (int r = 0; r < R; ++r) prw[w][r] = a[r];
+        }
+        wave_fence();
+        if (w == kb) a[kr] = cplx{0.0, 0.0};
+  #pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const cplx P = (w == kb && r == kr) ? cplx{1.0, 0.0} : prw[w][r];
+          a[r] = cplx{keep * a[r].re - (g.re * P.re - g.im * P.im),
+                      keep * a[r].im - (g.re * P.im + g.im * P.re)};
+        }
+        wave_fence();
+        // the row at position k and the pivot row exchange positions
+        if (pos == k)
+          pos = pp;
+        else if (lane == p)
+          pos = k;
+      }
+    }
+  }
+
+  // inv(A) = inv(P A) P, with P the row order built above: column c of the
+  // eliminated matrix is column invp[c] of the result, row i goes to row pos
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < n; ++c) perm[c] = c;
+    for (int k = n - 1; k >= 0; --k) {
+      const int t = perm[k];
+      perm[k] = perm[pivpos[k]];
+      perm[pivpos[k]] = t;
+    }
+    for (int c = 0; c < n; ++c) invp[perm[c]] = c;
+  }
+  if (lane == 0 && singular && info != nullptr)
+    atomicMin(info, ((unsigned long long)(unsigned)tag << 32) | (unsigned long long)(unsigned)z);
+  __syncthreads();
+  if (lane < n) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int c = c0 + r;
+      if (c < n)
+        *reinterpret_cast<double2*>(A + (long long)pos * lda + invp[c]) =
+            double2{a[r].re, a[r].im};
+    }
+  }
+}
+
+// Leaf kernel choice: the row-layout register kernel with 8 waves unless
+// SINV_LEAF_KERNEL names another ("lds": the LDS-resident kernel, "rows4":
+// the register kernel with 4 waves).  Read once per process.
+int leaf_variant() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("SINV_LEAF_KERNEL");
+    v = 2;
+    if (e != nullptr && strcmp(e, "lds") == 0) v = 0;
+    if (e != nullptr && strcmp(e, "rows4") == 0) v = 1;
+  }
+  return v;
+}
+
 // Split so that the recursion's GEMM dimensions stay multiples of the 64-wide
 // GEMM tile where possible: the largest multiple of 64 not above n/2, else a
 // multiple of 16 near n/2.
@@ -161,8 +366,20 @@ hipError_t launch_inverse(cplx* A, int n, int lda, long long s_in, long long s_o
   if (n <= 0 || inner <= 0 || outer <= 0) return hipSuccess;
   if (n <= LEAF) {
     const int pid = prof_open(s);
-    hipLaunchKernelGGL(inv_leaf_kernel, dim3((unsigned)(inner * outer)), dim3(256), 0, s, A, n,
-                       lda, s_in, s_out, inner, info, tag);
+    const dim3 grid((unsigned)(inner * outer));
+    switch (leaf_variant()) {
+      case 0:
+        hipLaunchKernelGGL(inv_leaf_kernel, grid, dim3(256), 0, s, A, n, lda, s_in, s_out, inner,
+                           info, tag);
+        break;
+      case 2:
+        hipLaunchKernelGGL(inv_leaf_kernel_rows<8>, grid, dim3(512), 0, s, A, n, lda, s_in,
+                           s_out, inner, info, tag);
+        break;
+      default:
+        hipLaunchKernelGGL(inv_leaf_kernel_rows<4>, grid, dim3(256), 0, s, A, n, lda, s_in,
+                           s_out, inner, info, tag);
+    }
     prof_close(pid, s, PROF_INV_LEAF, 8.0 * n * (double)n * n * inner * (double)outer,
                2.0 * sizeof(cplx) * n * (double)n * inner * (double)outer);
     return hipGetLastError();
