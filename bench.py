@@ -160,12 +160,12 @@ def main():
         return 0
 
     def timed(plans, steps, warmup, tag):
+        # the block-size groups are independent: run them concurrently, one stream each
+        pset = core.PlanSet(plans)
         for _ in range(warmup):
-            for p in plans:
-                p.run()
+            pset.run()
         torch.cuda.synchronize()
-        for p in plans:
-            p.check()
+        pset.check()
         # roctx range: lets a rocprofv3 --marker-trace run select this region's dispatches
         torch.cuda.nvtx.range_push(f"sinv_bench_timed:{tag}")
         t0 = time.perf_counter()
@@ -173,8 +173,7 @@ def main():
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
         for _ in range(steps):
-            for p in plans:
-                p.run()
+            pset.run()
         ev1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
@@ -188,10 +187,14 @@ def main():
         ms, wall = timed(plans, args.steps, args.warmup, prec)
         log(f"[bench] batched {prec}: {ms:.1f} ms/step (wall {wall * 1e3:.1f}), "
             f"{solves / (ms / 1e3):.1f} solves/s, {alg_flops / (ms / 1e3) / 1e12:.2f} TFLOP/s")
+        # kernel-level figures from one more step with the groups in sequence, so each
+        # kernel's event-bracketed time is its own (roctx range for a rocprofv3 cross-check)
+        torch.cuda.nvtx.range_push(f"sinv_bench_profiled:{prec}")
         with _native.LaunchProfiler() as prof:
             for p in plans:
                 p.run()
             torch.cuda.synchronize()
+        torch.cuda.nvtx.range_pop()
         results[prec] = dict(ms=ms, prof=prof.stats, bytes=prof.bytes)
         del plans
         torch.cuda.synchronize()

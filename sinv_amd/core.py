@@ -188,6 +188,46 @@ class RgfPlan:
         return g
 
 
+class PlanSet:
+    """Independent plans (e.g. one ``RgfPlan`` per block size) run concurrently:
+    ``run()`` forks from the current stream onto one stream per plan and joins
+    back, so one plan's latency-bound launches (leaf inverses, small GEMM
+    tails) overlap another's GEMMs.  Streams have the default priority.
+    Nothing is allocated per ``run()``; results are ready once the current
+    stream reaches the join."""
+
+    def __init__(self, plans):
+        self.plans = list(plans)
+        self.streams = [torch.cuda.Stream(device=device()) for _ in self.plans]
+
+    def run(self):
+        cur = torch.cuda.current_stream()
+        for st, p in zip(self.streams, self.plans):
+            st.wait_stream(cur)
+            with torch.cuda.stream(st):
+                p.run()
+        for st in self.streams:
+            cur.wait_stream(st)
+
+    def check(self):
+        for p in self.plans:
+            p.check()
+
+    def capture(self, warmup=1):
+        """Capture one concurrent ``run()`` into a HIP graph; the fork/join
+        across the plan streams is recorded as graph dependencies."""
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for _ in range(warmup):
+                self.run()
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run()
+        return g
+
+
 def bcr_batched(diag, upper, lower, *, shift=None, dscale=1.0, oscale=1.0, return_fill=False,
                 check=True):
     """BCR-serial selected inversion of a batch of instances (bcr_serial.py:19-466).

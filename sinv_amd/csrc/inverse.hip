@@ -164,10 +164,10 @@ __device__ __forceinline__ void amax_step(double& m, int& key) {
                                              false);
   const int ok = __builtin_amdgcn_update_dpp(key, key, CTRL, ROWS, 0xF, false);
   const double om = __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-  if (om > m || (om == m && ok < key)) {
-    m = om;
-    key = ok;
-  }
+  // bitwise, not short-circuit: keeps the step free of exec-mask branches
+  const bool take = (om > m) | ((om == m) & (ok < key));
+  m = take ? om : m;
+  key = take ? ok : key;
 }
 
 __device__ __forceinline__ void wave_fence() {
