@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--chunk", type=int, default=64, help="energies per adjoint launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the batch=1 FP64 sample")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the GEMM tile autotuner (untimed, before the timed steps)")
     ap.add_argument("--traffic-pass", action="store_true",
                     help="run one FP64 step and a 2 GiB calibration copy (neg kernel), untimed, "
                          "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs; prints no bench line")
@@ -144,6 +146,9 @@ def main():
 
     if args.traffic_pass:
         plans = make_plans("fp64")
+        if not args.no_tune:  # count the same tile configurations the timed bench runs
+            from sinv_amd import autotune
+            autotune.tune(lambda: [p.run() for p in plans], reps=2)
         for p in plans:
             p.run()
         torch.cuda.synchronize()
@@ -181,9 +186,19 @@ def main():
         return ev0.elapsed_time(ev1) / steps, wall / steps
 
     results = {}
+    tuned = {}
     for prec in ("fp64", "mixed"):
         log(f"[bench] {wl['workload']}: batched {prec}: allocating plans")
         plans = make_plans(prec)
+        if not args.no_tune:
+            # GEMM tile configs per shape, timed on scratch operands (outside the timed region)
+            from sinv_amd import autotune
+            t0 = time.perf_counter()
+            table = autotune.tune(lambda: [p.run() for p in plans], reps=2)
+            tuned[prec] = dict(shapes=len(table), seconds=round(time.perf_counter() - t0, 2),
+                               chosen={str(c): sum(1 for v in table.values() if v["best"] == c)
+                                       for c, _ in autotune.configs()})
+            log(f"[bench] autotuned {len(table)} GEMM shapes ({prec}) in {tuned[prec]['seconds']} s")
         ms, wall = timed(plans, args.steps, args.warmup, prec)
         log(f"[bench] batched {prec}: {ms:.1f} ms/step (wall {wall * 1e3:.1f}), "
             f"{solves / (ms / 1e3):.1f} solves/s, {alg_flops / (ms / 1e3) / 1e12:.2f} TFLOP/s")
@@ -340,6 +355,7 @@ def main():
         "speedup_vs_plain_fp64": (round(solves / (ms / 1e3) / plain["solves_per_s"], 2) if plain else None),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "gemm_autotune": tuned or None,
         "kernel_time": {k: dict(ms=round(v[0], 3), launches=v[1], tflops=round(v[2] / (v[0] / 1e3) / 1e12, 3)
                                 if v[0] > 0 else 0.0) for k, v in results[head]["prof"].items() if v[1]},
     }
@@ -389,7 +405,15 @@ def bench_adjoint(args):
         return loss, grad
 
     out = {}
+    tuned = {}
     for prec in ("fp64", "mixed"):
+        if not args.no_tune:
+            # every chunk issues the same GEMM shapes: tune on one (untimed)
+            from sinv_amd import autotune
+            t0 = time.perf_counter()
+            table = autotune.tune(lambda: step(prec, limit=args.chunk), reps=2)
+            tuned[prec] = dict(shapes=len(table), seconds=round(time.perf_counter() - t0, 2))
+            log(f"[bench] autotuned {len(table)} GEMM shapes ({prec}) in {tuned[prec]['seconds']} s")
         log(f"[bench] adjoint workload {prec}: warmup")
         torch.cuda.reset_peak_memory_stats()
         for _ in range(args.warmup):
@@ -438,6 +462,7 @@ def bench_adjoint(args):
         "executed_tflops": round(executed / (ms / 1e3) / 1e12, 3),
         "forward_algorithmic_tflops": round(fwd_flops / (ms / 1e3) / 1e12, 3),
         "peak_hbm_gb": round(out[head]["peak_gb"], 2),
+        "gemm_autotune": tuned or None,
         "modes": {
             "fp64": dict(solves_per_s=round(nE / (out["fp64"]["ms"] / 1e3), 3),
                          ms_per_step=round(out["fp64"]["ms"], 3), peak_hbm_gb=round(out["fp64"]["peak_gb"], 2)),

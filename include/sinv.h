@@ -239,6 +239,29 @@ int sinv_zgemm_batched(int M, int N, int K, int opA, int opB, sinv_c128 alpha,
                        const void* shift, int batch, int precision, void* stream);
 
 /* ------------------------------------------------------------------------
+ * GEMM tile configurations and tuned shapes (no counterpart in the reference:
+ * numpy's BLAS picks its own blocking).  Every complex GEMM the library issues
+ * (sinv_zgemm_batched and the ones inside the RGF, BCR, PSR, adjoint and
+ * inverse drivers) runs one of sinv_gemm_config_count() tile configurations;
+ * sinv_gemm_config_tile gives its workgroup tile (bm x bn).  The config is
+ * chosen per launch: a forced config if set (sinv_gemm_force_config, -1
+ * clears), else the tuned entry of the exact shape (M, N, K, batch, opA, opB,
+ * C0 present, precision) if one was set with sinv_gemm_tune_set, else a fill
+ * heuristic.  sinv_gemm_select returns the config a launch of that shape
+ * would use.  The table is process-wide and unsynchronised: set it before
+ * launching the work it applies to, not concurrently with launches.  The
+ * autotuner that fills it is sinv_amd.autotune.
+ * ------------------------------------------------------------------------ */
+int sinv_gemm_config_count(void);
+int sinv_gemm_config_tile(int cfg, int* bm, int* bn);
+int sinv_gemm_force_config(int cfg);
+int sinv_gemm_tune_set(int M, int N, int K, long long batch, int opA, int opB, int has_c0,
+                       int precision, int cfg);
+int sinv_gemm_tune_clear(void);
+int sinv_gemm_select(int M, int N, int K, long long batch, int opA, int opB, int has_c0,
+                     int precision);
+
+/* ------------------------------------------------------------------------
  * Launch profiler (no counterpart in the reference, which has no timing code).
  * Between sinv_prof_begin and sinv_prof_end every kernel launch of the GEMM
  * and leaf-inverse kernels is bracketed by hipEvents on its own stream.
@@ -253,6 +276,11 @@ int sinv_zgemm_batched(int M, int N, int K, int opA, int opB, sinv_c128 alpha,
  * ------------------------------------------------------------------------ */
 int sinv_prof_begin(int max_launches);
 int sinv_prof_end(double* ms, long long* launches, double* flops, double* bytes, int nkinds);
+/* Per-launch records of the last region closed by sinv_prof_end, in launch
+ * order: kind, 5 dims (GEMM: M, N, K, batch, opA | opB << 2 | (C0 != NULL) << 4;
+ * zeros for other kinds) and the event-bracketed time in ms.  Copies at most
+ * `cap` records and returns how many there are. */
+int sinv_prof_records(int cap, int* kind, long long* dims, float* ms);
 
 #ifdef __cplusplus
 }

@@ -128,6 +128,21 @@ def lib():
     L.sinv_prof_end.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_ll),
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                 _i]
+    L.sinv_gemm_config_count.restype = _i
+    L.sinv_gemm_config_count.argtypes = []
+    L.sinv_gemm_config_tile.restype = _i
+    L.sinv_gemm_config_tile.argtypes = [_i, ctypes.POINTER(_i), ctypes.POINTER(_i)]
+    L.sinv_gemm_force_config.restype = _i
+    L.sinv_gemm_force_config.argtypes = [_i]
+    L.sinv_gemm_tune_set.restype = _i
+    L.sinv_gemm_tune_set.argtypes = [_i, _i, _i, _ll, _i, _i, _i, _i, _i]
+    L.sinv_gemm_tune_clear.restype = _i
+    L.sinv_gemm_tune_clear.argtypes = []
+    L.sinv_gemm_select.restype = _i
+    L.sinv_gemm_select.argtypes = [_i, _i, _i, _ll, _i, _i, _i, _i]
+    L.sinv_prof_records.restype = _i
+    L.sinv_prof_records.argtypes = [_i, ctypes.POINTER(_i), ctypes.POINTER(_ll),
+                                    ctypes.POINTER(ctypes.c_float)]
     if hasattr(L, "sinv_bcr_batched"):
         L.sinv_bcr_workspace_bytes.restype = ctypes.c_size_t
         L.sinv_bcr_workspace_bytes.argtypes = [ctypes.POINTER(BcrDesc)]
@@ -200,3 +215,14 @@ class LaunchProfiler:
         self.stats = {k: (ms[i], cnt[i], fl[i]) for i, k in enumerate(PROF_KINDS)}
         self.bytes = {k: by[i] for i, k in enumerate(PROF_KINDS)}
         return False
+
+    def records(self):
+        """Per-launch (kind, (M, N, K, batch, flags), ms) of the profiled region."""
+        L = lib()
+        n = L.sinv_prof_records(0, None, None, None)
+        kind = (ctypes.c_int * max(n, 1))()
+        dims = (ctypes.c_longlong * max(5 * n, 1))()
+        ms = (ctypes.c_float * max(n, 1))()
+        L.sinv_prof_records(n, kind, dims, ms)
+        return [(PROF_KINDS[kind[i]] if 0 <= kind[i] < len(PROF_KINDS) else "other",
+                 tuple(dims[5 * i:5 * i + 5]), ms[i]) for i in range(n)]

@@ -73,10 +73,7 @@ __device__ __forceinline__ int acc_row<float>(int lane, int v) {
 template <typename R>
 constexpr bool kThreeM = sizeof(R) == 8;
 
-constexpr int BM = 64;
-constexpr int BN = 64;
 constexpr int BK = 16;
-constexpr int NTHR = 256;
 constexpr int PAD = 1;
 
 struct TileMap {
@@ -85,6 +82,7 @@ struct TileMap {
 
 // XCD-aware remap: hardware places workgroup `bid` on XCD bid % 8.  Give each
 // XCD a contiguous range of logical tiles.
+template <int BMt, int BNt>
 __device__ __forceinline__ TileMap map_tile(int tiles_m, int tiles_n, int inner) {
   const int total = gridDim.x;
   int bid = blockIdx.x;
@@ -98,8 +96,8 @@ __device__ __forceinline__ TileMap map_tile(int tiles_m, int tiles_n, int inner)
   TileMap m;
   m.z_out = z / inner;
   m.z_in = z - m.z_out * inner;
-  m.m0 = (t / tiles_n) * BM;
-  m.n0 = (t % tiles_n) * BN;
+  m.m0 = (t / tiles_n) * BMt;
+  m.n0 = (t % tiles_n) * BNt;
   return m;
 }
 
@@ -108,87 +106,125 @@ __device__ __forceinline__ cplx ld_c(const cplx* p) {
   return cplx{v.x, v.y};
 }
 
-template <typename R>
-__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(2)))
-void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
-  __shared__ CR<R> As2[2][BK][BM + PAD];
-  __shared__ CR<R> Bs2[2][BK][BN + PAD];
+// Tile configuration: WM x WN waves, each owning TM x TN MFMA tiles of 16 x 16;
+// workgroup tile (16 WM TM) x (16 WN TN).  OCC is the occupancy the register
+// budget is sized for (waves per SIMD).
+template <int WM_, int WN_, int TM_, int TN_, int OCC_>
+struct Cfg {
+  static constexpr int WM = WM_, WN = WN_, TM = TM_, TN = TN_, OCC = OCC_;
+  static constexpr int BM = 16 * WM * TM, BN = 16 * WN * TN;
+  static constexpr int NT = 64 * WM * WN;
+  // load groups: VA (VB) elements consecutive in memory, GA (GB) groups per thread
+  static constexpr int VA = BM * BK / NT >= 4 ? 4 : BM * BK / NT;
+  static constexpr int VB = BN * BK / NT >= 4 ? 4 : BN * BK / NT;
+  static constexpr int GA = BM * BK / VA / NT;
+  static constexpr int GB = BN * BK / VB / NT;
+  static_assert(VA >= 1 && VB >= 1 && GA * VA * NT == BM * BK && GB * VB * NT == BN * BK,
+                "tile must split evenly into load groups");
+};
 
-  const TileMap tm = map_tile(tiles_m, tiles_n, g.inner);
+template <typename R, typename C>
+__global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::OCC)))
+void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
+  constexpr int BMt = C::BM, BNt = C::BN, NT = C::NT, TM = C::TM, TN = C::TN;
+  constexpr int GA = C::GA, GB = C::GB, VA = C::VA, VB = C::VB;
+  __shared__ CR<R> As2[2][BK][BMt + PAD];
+  __shared__ CR<R> Bs2[2][BK][BNt + PAD];
+
+  const TileMap tm = map_tile<BMt, BNt>(tiles_m, tiles_n, g.inner);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / C::WN, wn = wave % C::WN;
 
   const cplx* A = g.A + tm.z_in * g.sA_in + tm.z_out * g.sA_out;
   const cplx* B = g.B + tm.z_in * g.sB_in + tm.z_out * g.sB_out;
 
-  // per-thread global-load coordinates (4 consecutive complex each)
-  // op(A) tile is BM x BK; op(B) tile is BK x BN.
-  const bool a_kcontig = (g.opA == OP_N);  // A row-major M x K: contiguous along k
-  const bool b_ncontig = (g.opB == OP_N);  // B row-major K x N: contiguous along n
+  // op(A) tile is BM x BK, op(B) tile is BK x BN; each thread loads groups of
+  // VA (VB) elements consecutive in memory (along k for A stored M x K, along m
+  // for A stored K x M; along n for B stored K x N, along k for B stored N x K)
+  const bool a_kcontig = (g.opA == OP_N);
+  const bool b_ncontig = (g.opB == OP_N);
   const double a_conj = (g.opA == OP_C) ? -1.0 : 1.0;
   const double b_conj = (g.opB == OP_C) ? -1.0 : 1.0;
 
-  int a_r, a_c;  // (m, k) of first element in tile coordinates
-  if (a_kcontig) { a_r = tid >> 2; a_c = (tid & 3) * 4; }
-  else           { a_c = tid >> 4; a_r = (tid & 15) * 4; }
-  int b_r, b_c;  // (k, n)
-  if (b_ncontig) { b_r = tid >> 4; b_c = (tid & 15) * 4; }
-  else           { b_c = tid >> 2; b_r = (tid & 3) * 4; }
+  int a_r[GA], a_c[GA];  // (m, k) of each group's first element, tile coordinates
+#pragma unroll
+  for (int q = 0; q < GA; ++q) {
+    const int L = tid + q * NT;
+    if (a_kcontig) { a_r[q] = L / (BK / VA); a_c[q] = (L % (BK / VA)) * VA; }
+    else           { a_c[q] = L / (BMt / VA); a_r[q] = (L % (BMt / VA)) * VA; }
+  }
+  int b_r[GB], b_c[GB];  // (k, n)
+#pragma unroll
+  for (int q = 0; q < GB; ++q) {
+    const int L = tid + q * NT;
+    if (b_ncontig) { b_r[q] = L / (BNt / VB); b_c[q] = (L % (BNt / VB)) * VB; }
+    else           { b_c[q] = L / (BK / VB); b_r[q] = (L % (BK / VB)) * VB; }
+  }
 
-  cplx ra[4], rb[4];
-  bool oka[4], okb[4];
+  cplx ra[GA][VA], rb[GB][VB];
+  bool oka[GA][VA], okb[GB][VB];
 
   // Branch-free tile loads: an out-of-range element is read from (0, 0), which
-  // always exists, and zeroed when staged into LDS.  All eight loads of a
-  // chunk are then in flight together instead of each behind its own branch.
+  // always exists, and zeroed when staged into LDS.  All loads of a chunk are
+  // then in flight together instead of each behind its own branch.
   auto load_tiles = [&](int k0) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int m = tm.m0 + a_r + (a_kcontig ? 0 : e);
-      const int k = k0 + a_c + (a_kcontig ? e : 0);
-      oka[e] = (m < g.M) && (k < g.K);
-      const long long mm = oka[e] ? m : 0, kk = oka[e] ? k : 0;
-      ra[e] = a_kcontig ? ld_c(A + mm * g.lda + kk) : ld_c(A + kk * g.lda + mm);
-    }
+    for (int q = 0; q < GA; ++q)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int k = k0 + b_r + (b_ncontig ? 0 : e);
-      const int n = tm.n0 + b_c + (b_ncontig ? e : 0);
-      okb[e] = (k < g.K) && (n < g.N);
-      const long long kk = okb[e] ? k : 0, nn = okb[e] ? n : 0;
-      rb[e] = b_ncontig ? ld_c(B + kk * g.ldb + nn) : ld_c(B + nn * g.ldb + kk);
-    }
+      for (int e = 0; e < VA; ++e) {
+        const int m = tm.m0 + a_r[q] + (a_kcontig ? 0 : e);
+        const int k = k0 + a_c[q] + (a_kcontig ? e : 0);
+        oka[q][e] = (m < g.M) && (k < g.K);
+        const long long mm = oka[q][e] ? m : 0, kk = oka[q][e] ? k : 0;
+        ra[q][e] = a_kcontig ? ld_c(A + mm * g.lda + kk) : ld_c(A + kk * g.lda + mm);
+      }
+#pragma unroll
+    for (int q = 0; q < GB; ++q)
+#pragma unroll
+      for (int e = 0; e < VB; ++e) {
+        const int k = k0 + b_r[q] + (b_ncontig ? 0 : e);
+        const int n = tm.n0 + b_c[q] + (b_ncontig ? e : 0);
+        okb[q][e] = (k < g.K) && (n < g.N);
+        const long long kk = okb[q][e] ? k : 0, nn = okb[q][e] ? n : 0;
+        rb[q][e] = b_ncontig ? ld_c(B + kk * g.ldb + nn) : ld_c(B + nn * g.ldb + kk);
+      }
   };
   auto store_tiles = [&](int buf) {
     auto& As = As2[buf];
     auto& Bs = Bs2[buf];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int m = a_r + (a_kcontig ? 0 : e);
-      const int k = a_c + (a_kcontig ? e : 0);
-      const double re = oka[e] ? ra[e].re : 0.0, im = oka[e] ? ra[e].im * a_conj : 0.0;
-      As[k][m] = CR<R>{(R)re, (R)im};
-    }
+    for (int q = 0; q < GA; ++q)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int k = b_r + (b_ncontig ? 0 : e);
-      const int n = b_c + (b_ncontig ? e : 0);
-      const double re = okb[e] ? rb[e].re : 0.0, im = okb[e] ? rb[e].im * b_conj : 0.0;
-      Bs[k][n] = CR<R>{(R)re, (R)im};
-    }
+      for (int e = 0; e < VA; ++e) {
+        const int m = a_r[q] + (a_kcontig ? 0 : e);
+        const int k = a_c[q] + (a_kcontig ? e : 0);
+        const double re = oka[q][e] ? ra[q][e].re : 0.0;
+        const double im = oka[q][e] ? ra[q][e].im * a_conj : 0.0;
+        As[k][m] = CR<R>{(R)re, (R)im};
+      }
+#pragma unroll
+    for (int q = 0; q < GB; ++q)
+#pragma unroll
+      for (int e = 0; e < VB; ++e) {
+        const int k = b_r[q] + (b_ncontig ? 0 : e);
+        const int n = b_c[q] + (b_ncontig ? e : 0);
+        const double re = okb[q][e] ? rb[q][e].re : 0.0;
+        const double im = okb[q][e] ? rb[q][e].im * b_conj : 0.0;
+        Bs[k][n] = CR<R>{(R)re, (R)im};
+      }
   };
 
   using V = typename Acc<R>::V;
   // 3M (Karatsuba) complex product: T1 = Ar Br, T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi);
   // Re = T1 - T2, Im = T3 - T1 - T2.  Three real MFMAs per complex product
   // instead of four, normwise error of the same order as the 4-product form.
-  V t1[2][2], t2[2][2], t3[2][2];
+  V t1[TM][TN], t2[TM][TN], t3[TM][TN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < TN; ++j) {
       t1[i][j] = V{0, 0, 0, 0};
       t2[i][j] = V{0, 0, 0, 0};
       t3[i][j] = V{0, 0, 0, 0};
@@ -202,49 +238,47 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kr = kk + (lane >> 4);
-      CR<R> a[2], b[2];
+      CR<R> a[TM], b[TN];
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        a[q] = As[kr][wm * 32 + q * 16 + (lane & 15)];
-        b[q] = Bs[kr][wn * 32 + q * 16 + (lane & 15)];
-      }
+      for (int q = 0; q < TM; ++q) a[q] = As[kr][wm * TM * 16 + q * 16 + (lane & 15)];
+#pragma unroll
+      for (int q = 0; q < TN; ++q) b[q] = Bs[kr][wn * TN * 16 + q * 16 + (lane & 15)];
       if constexpr (kThreeM<R>) {
-        R as[2], bs[2];
+        R as[TM], bs[TN];
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          as[q] = a[q].re + a[q].im;
-          bs[q] = b[q].re + b[q].im;
-        }
+        for (int q = 0; q < TM; ++q) as[q] = a[q].re + a[q].im;
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int q = 0; q < TN; ++q) bs[q] = b[q].re + b[q].im;
 #pragma unroll
-          for (int j = 0; j < 2; ++j) t1[i][j] = mfma(a[i].re, b[j].re, t1[i][j]);
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+          for (int j = 0; j < TN; ++j) t1[i][j] = mfma(a[i].re, b[j].re, t1[i][j]);
 #pragma unroll
-          for (int j = 0; j < 2; ++j) t2[i][j] = mfma(a[i].im, b[j].im, t2[i][j]);
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+          for (int j = 0; j < TN; ++j) t2[i][j] = mfma(a[i].im, b[j].im, t2[i][j]);
 #pragma unroll
-          for (int j = 0; j < 2; ++j) t3[i][j] = mfma(as[i], bs[j], t3[i][j]);
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) t3[i][j] = mfma(as[i], bs[j], t3[i][j]);
       } else {
         // 4 real products: t1 = Re, t3 = Im (t2 unused)
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
-          for (int j = 0; j < 2; ++j) t1[i][j] = mfma(a[i].re, b[j].re, t1[i][j]);
+          for (int j = 0; j < TN; ++j) t1[i][j] = mfma(a[i].re, b[j].re, t1[i][j]);
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
-          for (int j = 0; j < 2; ++j) t3[i][j] = mfma(a[i].re, b[j].im, t3[i][j]);
+          for (int j = 0; j < TN; ++j) t3[i][j] = mfma(a[i].re, b[j].im, t3[i][j]);
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
-          for (int j = 0; j < 2; ++j) t1[i][j] = mfma(-a[i].im, b[j].im, t1[i][j]);
+          for (int j = 0; j < TN; ++j) t1[i][j] = mfma(-a[i].im, b[j].im, t1[i][j]);
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
-          for (int j = 0; j < 2; ++j) t3[i][j] = mfma(a[i].im, b[j].re, t3[i][j]);
+          for (int j = 0; j < TN; ++j) t3[i][j] = mfma(a[i].im, b[j].re, t3[i][j]);
       }
     }
   };
@@ -262,20 +296,20 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
   }
 
   // epilogue
-  cplx* C = g.C + tm.z_in * g.sC_in + tm.z_out * g.sC_out;
+  cplx* Cp = g.C + tm.z_in * g.sC_in + tm.z_out * g.sC_out;
   const bool use_c0 = g.C0 != nullptr && (g.beta.re != 0.0 || g.beta.im != 0.0);
   const cplx* C0 = use_c0 ? g.C0 + tm.z_in * g.sC0_in + tm.z_out * g.sC0_out : nullptr;
   cplx sh{0.0, 0.0};
   const bool use_shift = g.shift != nullptr;
   if (use_shift) sh = g.shift[tm.z_out];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = tm.m0 + wm * 32 + i * 16 + acc_row<R>(lane, v);
-        const int col = tm.n0 + wn * 32 + j * 16 + (lane & 15);
+        const int row = tm.m0 + wm * TM * 16 + i * 16 + acc_row<R>(lane, v);
+        const int col = tm.n0 + wn * TN * 16 + j * 16 + (lane & 15);
         if (row < g.M && col < g.N) {
           double xr, xi;
           if constexpr (kThreeM<R>) {
@@ -297,10 +331,76 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
             outr += sh.re;
             outi += sh.im;
           }
-          *reinterpret_cast<double2*>(C + (long long)row * g.ldc + col) = double2{outr, outi};
+          *reinterpret_cast<double2*>(Cp + (long long)row * g.ldc + col) = double2{outr, outi};
         }
       }
 }
+
+// The tile configurations the launcher can choose from (index = config id).
+// Every wave of the larger tiles keeps the 32 x 32 (2 x 2 MFMA tiles) shape of
+// the default one; larger tiles add waves, not registers.
+using Cfg64 = Cfg<2, 2, 2, 2, 2>;     // 0: 64 x 64, 4 waves
+using Cfg32w4 = Cfg<2, 2, 1, 1, 4>;   // 1: 32 x 32, 4 waves of 16 x 16
+using Cfg32w1 = Cfg<1, 1, 2, 2, 4>;   // 2: 32 x 32, 1 wave
+using Cfg128x64 = Cfg<4, 2, 2, 2, 2>; // 3: 128 x 64, 8 waves
+using Cfg64x128 = Cfg<2, 4, 2, 2, 2>; // 4: 64 x 128, 8 waves
+constexpr int kNumCfg = 5;
+constexpr int kCfgBM[kNumCfg] = {Cfg64::BM, Cfg32w4::BM, Cfg32w1::BM, Cfg128x64::BM,
+                                 Cfg64x128::BM};
+constexpr int kCfgBN[kNumCfg] = {Cfg64::BN, Cfg32w4::BN, Cfg32w1::BN, Cfg128x64::BN,
+                                 Cfg64x128::BN};
+
+template <typename R, typename C>
+void launch_cfg(const GemmArgs& g, long long batch, hipStream_t s) {
+  const int tiles_m = (g.M + C::BM - 1) / C::BM;
+  const int tiles_n = (g.N + C::BN - 1) / C::BN;
+  hipLaunchKernelGGL((gemm_kernel<R, C>), dim3((unsigned)((long long)tiles_m * tiles_n * batch)),
+                     dim3(C::NT), 0, s, g, tiles_m, tiles_n);
+}
+
+template <typename R>
+void launch_by_id(int id, const GemmArgs& g, long long batch, hipStream_t s) {
+  switch (id) {
+    case 1: launch_cfg<R, Cfg32w4>(g, batch, s); break;
+    case 2: launch_cfg<R, Cfg32w1>(g, batch, s); break;
+    case 3: launch_cfg<R, Cfg128x64>(g, batch, s); break;
+    case 4: launch_cfg<R, Cfg64x128>(g, batch, s); break;
+    default: launch_cfg<R, Cfg64>(g, batch, s); break;
+  }
+}
+
+// ---- tuned configurations: (shape key) -> config id, written by
+// sinv_gemm_tune_set, read at every launch.  Fixed-size open table; not
+// synchronised: set entries before launching work that uses them.
+struct TuneEntry {
+  int M, N, K, opA, opB, c0, prec, cfg;
+  long long batch;
+  bool used;
+};
+constexpr int kTuneSlots = 1024;
+TuneEntry g_tune[kTuneSlots];
+int g_tune_count = 0;
+
+int tune_find(const GemmArgs& g, long long batch, int prec) {
+  const int c0 = g.C0 != nullptr ? 1 : 0;
+  for (int i = 0; i < g_tune_count; ++i) {
+    const TuneEntry& e = g_tune[i];
+    if (e.used && e.M == g.M && e.N == g.N && e.K == g.K && e.batch == batch && e.opA == g.opA &&
+        e.opB == g.opB && e.c0 == c0 && e.prec == prec)
+      return e.cfg;
+  }
+  return -1;
+}
+
+// Untuned shapes: the 64 x 64 tile, unless its grid leaves the chip under half
+// full (under 2 workgroups per CU), where 32 x 32 tiles of 4 waves give 4x the
+// workgroups.
+int default_cfg(const GemmArgs& g, long long batch) {
+  const long long t64 = (long long)((g.M + 63) / 64) * ((g.N + 63) / 64) * batch;
+  return t64 < 512 ? 1 : 0;
+}
+
+int g_force_cfg = -1;  // >= 0: every launch uses this config (tuning runs)
 
 // ---------------------------------------------------------------------------
 // elementwise helpers (memory-bound; one thread per complex element)
@@ -387,18 +487,21 @@ hipError_t launch_gemm(const GemmArgs& g, Prec p, hipStream_t s) {
     return launch_axpby_shift(g.M, g.N, g.beta, g.C0, g.ldc0, g.sC0_in, g.sC0_out, g.C, g.ldc,
                               g.sC_in, g.sC_out, g.shift, g.inner, g.outer, s);
   }
-  const int tiles_m = (g.M + BM - 1) / BM;
-  const int tiles_n = (g.N + BN - 1) / BN;
-  const long long blocks = (long long)tiles_m * tiles_n * g.inner * g.outer;
+  const long long batch = (long long)g.inner * g.outer;
+  const int prec = p == Prec::F64 ? SINV_PREC_FP64 : SINV_PREC_MIXED;
+  int id = g_force_cfg;
+  if (id < 0) id = tune_find(g, batch, prec);
+  if (id < 0) id = default_cfg(g, batch);
+  const long long blocks = (long long)((g.M + kCfgBM[id] - 1) / kCfgBM[id]) *
+                           ((g.N + kCfgBN[id] - 1) / kCfgBN[id]) * batch;
   if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
   const int pid = prof_open(s);
   if (p == Prec::F64)
-    hipLaunchKernelGGL(gemm_kernel<double>, dim3((unsigned)blocks), dim3(NTHR), 0, s, g, tiles_m,
-                       tiles_n);
+    launch_by_id<double>(id, g, batch, s);
   else
-    hipLaunchKernelGGL(gemm_kernel<float>, dim3((unsigned)blocks), dim3(NTHR), 0, s, g, tiles_m,
-                       tiles_n);
-  const double nb = g.inner * (double)g.outer;
+    launch_by_id<float>(id, g, batch, s);
+  const double nb = (double)batch;
+  prof_shape(pid, g.M, g.N, g.K, batch, g.opA | (g.opB << 2) | (g.C0 != nullptr ? 16 : 0));
   const double elems = (double)g.M * g.K + (double)g.K * g.N +
                        (double)g.M * g.N * (g.C0 != nullptr ? 2.0 : 1.0);
   prof_close(pid, s, p == Prec::F64 ? PROF_GEMM_F64 : PROF_GEMM_F32,
@@ -495,4 +598,70 @@ extern "C" int sinv_zgemm_batched(int M, int N, int K, int opA, int opB, sinv_c1
   SINV_HIP_TRY(launch_gemm(g, precision == SINV_PREC_FP64 ? Prec::F64 : Prec::F32,
                            (hipStream_t)stream));
   return SINV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// GEMM tile configurations and the tuned-shape table (C ABI)
+extern "C" int sinv_gemm_config_count(void) { return sinv::kNumCfg; }
+
+extern "C" int sinv_gemm_config_tile(int cfg, int* bm, int* bn) {
+  using namespace sinv;
+  if (cfg < 0 || cfg >= kNumCfg || bm == nullptr || bn == nullptr) {
+    set_error("sinv_gemm_config_tile: bad config id");
+    return SINV_E_INVAL;
+  }
+  *bm = kCfgBM[cfg];
+  *bn = kCfgBN[cfg];
+  return SINV_OK;
+}
+
+extern "C" int sinv_gemm_force_config(int cfg) {
+  using namespace sinv;
+  if (cfg < -1 || cfg >= kNumCfg) {
+    set_error("sinv_gemm_force_config: bad config id");
+    return SINV_E_INVAL;
+  }
+  g_force_cfg = cfg;
+  return SINV_OK;
+}
+
+extern "C" int sinv_gemm_tune_set(int M, int N, int K, long long batch, int opA, int opB,
+                                  int has_c0, int precision, int cfg) {
+  using namespace sinv;
+  if (cfg < 0 || cfg >= kNumCfg || M <= 0 || N <= 0 || K <= 0 || batch <= 0) {
+    set_error("sinv_gemm_tune_set: invalid argument");
+    return SINV_E_INVAL;
+  }
+  const int c0 = has_c0 ? 1 : 0;
+  for (int i = 0; i < g_tune_count; ++i) {
+    TuneEntry& e = g_tune[i];
+    if (e.M == M && e.N == N && e.K == K && e.batch == batch && e.opA == opA && e.opB == opB &&
+        e.c0 == c0 && e.prec == precision) {
+      e.cfg = cfg;
+      e.used = true;
+      return SINV_OK;
+    }
+  }
+  if (g_tune_count >= kTuneSlots) {
+    set_error("sinv_gemm_tune_set: table full");
+    return SINV_E_INVAL;
+  }
+  g_tune[g_tune_count++] = TuneEntry{M, N, K, opA, opB, c0, precision, cfg, batch, true};
+  return SINV_OK;
+}
+
+extern "C" int sinv_gemm_tune_clear(void) {
+  sinv::g_tune_count = 0;
+  return SINV_OK;
+}
+
+extern "C" int sinv_gemm_select(int M, int N, int K, long long batch, int opA, int opB,
+                                int has_c0, int precision) {
+  using namespace sinv;
+  GemmArgs g{};
+  g.M = M; g.N = N; g.K = K; g.opA = opA; g.opB = opB;
+  g.C0 = has_c0 ? reinterpret_cast<const cplx*>(16) : nullptr;  // only tested against null
+  if (g_force_cfg >= 0) return g_force_cfg;
+  const int id = tune_find(g, batch, precision);
+  return id >= 0 ? id : default_cfg(g, batch);
 }

@@ -16,6 +16,13 @@ struct Rec {
   int kind;
   double flops;
   double bytes;
+  long long dims[5];  // GEMM: M, N, K, batch, flags (opA | opB << 2 | has C0 << 4)
+};
+
+struct Last {
+  int kind;
+  long long dims[5];
+  float ms;
 };
 
 struct Prof {
@@ -23,6 +30,7 @@ struct Prof {
   size_t next = 0;
   std::vector<Rec> recs;
   bool overflow = false;
+  std::vector<Last> last;  // per-launch records of the last completed region
 };
 
 Prof g_prof;
@@ -38,6 +46,16 @@ int prof_open(hipStream_t s) {
   const int id = (int)g_prof.next++;
   (void)hipEventRecord(g_prof.recs[id].a, s);
   return id;
+}
+
+void prof_shape(int id, int M, int N, int K, long long batch, int flags) {
+  if (id < 0) return;
+  long long* d = g_prof.recs[id].dims;
+  d[0] = M;
+  d[1] = N;
+  d[2] = K;
+  d[3] = batch;
+  d[4] = flags;
 }
 
 void prof_close(int id, hipStream_t s, int kind, double flops, double bytes) {
@@ -64,6 +82,7 @@ extern "C" int sinv_prof_begin(int max_launches) {
     r.kind = -1;
     r.flops = 0.0;
     r.bytes = 0.0;
+    for (long long& x : r.dims) x = 0;
   }
   g_prof.next = 0;
   g_prof.overflow = false;
@@ -85,11 +104,14 @@ extern "C" int sinv_prof_end(double* ms, long long* launches, double* flops, dou
     if (bytes != nullptr) bytes[k] = 0.0;
   }
   int rc = SINV_OK;
+  g_prof.last.clear();
   for (size_t i = 0; i < g_prof.next; ++i) {
     Rec& r = g_prof.recs[i];
     if (hipEventSynchronize(r.b) != hipSuccess) rc = SINV_E_HIP;
     float t = 0.f;
     if (hipEventElapsedTime(&t, r.a, r.b) != hipSuccess) rc = SINV_E_HIP;
+    Last l{r.kind, {r.dims[0], r.dims[1], r.dims[2], r.dims[3], r.dims[4]}, t};
+    g_prof.last.push_back(l);
     if (r.kind >= 0 && r.kind < nkinds) {
       ms[r.kind] += t;
       launches[r.kind] += 1;
@@ -108,4 +130,14 @@ extern "C" int sinv_prof_end(double* ms, long long* launches, double* flops, dou
   }
   if (rc != SINV_OK) set_error("sinv_prof_end: event query failed");
   return rc;
+}
+
+extern "C" int sinv_prof_records(int cap, int* kind, long long* dims, float* ms) {
+  const int n = (int)g_prof.last.size();
+  for (int i = 0; i < n && i < cap; ++i) {
+    kind[i] = g_prof.last[i].kind;
+    for (int j = 0; j < 5; ++j) dims[5 * i + j] = g_prof.last[i].dims[j];
+    ms[i] = g_prof.last[i].ms;
+  }
+  return n;
 }

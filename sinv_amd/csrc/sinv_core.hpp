@@ -118,6 +118,8 @@ void set_error(const char* fmt, ...);
 enum ProfKind : int { PROF_GEMM_F64 = 0, PROF_GEMM_F32 = 1, PROF_INV_LEAF = 2, PROF_OTHER = 3 };
 int prof_open(hipStream_t s);
 void prof_close(int id, hipStream_t s, int kind, double flops, double bytes);
+// shape of a profiled GEMM launch (kept in the per-launch records)
+void prof_shape(int id, int M, int N, int K, long long batch, int flags);
 
 }  // namespace sinv
 

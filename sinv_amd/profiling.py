@@ -25,8 +25,13 @@ from collections import defaultdict
 
 
 def _short(name):
-    for key in ("gemm_kernel<double>", "gemm_kernel<float>", "inv_leaf_kernel", "axpby_shift_kernel",
-                "transpose_kernel", "fill_kernel", "fill_u64_kernel"):
+    # every tile configuration of the GEMM counts as one kernel per precision
+    for key, short in (("gemm_kernel<double", "gemm_kernel<double>"),
+                       ("gemm_kernel<float", "gemm_kernel<float>")):
+        if key in name:
+            return short
+    for key in ("inv_leaf_kernel", "axpby_shift_kernel", "transpose_kernel", "fill_u64_kernel",
+                "fill_kernel"):
         if key in name:
             return key
     return name.split("(")[0][-60:]

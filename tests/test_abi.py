@@ -61,3 +61,44 @@ def test_gemm_argument_validation(built_lib):
                                       ctypes.c_void_p(16), 4, 0, z, None, 4, 0,
                                       ctypes.c_void_p(16), 4, 0, None, 1, 0, None)
     assert rc == nat.SINV_E_INVAL
+
+
+def test_gemm_tune_table_host_side(built_lib):
+    """The tuned-shape table and the config queries are host-only state."""
+    import ctypes as ct
+    n = built_lib.sinv_gemm_config_count()
+    assert n >= 2
+    bm, bn = ct.c_int(), ct.c_int()
+    tiles = []
+    for c in range(n):
+        assert built_lib.sinv_gemm_config_tile(c, ct.byref(bm), ct.byref(bn)) == 0
+        tiles.append((bm.value, bn.value))
+    assert tiles[0] == (64, 64)
+    assert built_lib.sinv_gemm_config_tile(n, ct.byref(bm), ct.byref(bn)) != 0
+    key = (200, 120, 64, 7, 0, 2, 1, 0)
+    built_lib.sinv_gemm_tune_clear()
+    default = built_lib.sinv_gemm_select(*key)
+    other = (default + 1) % n
+    assert built_lib.sinv_gemm_tune_set(*key, other) == 0
+    assert built_lib.sinv_gemm_select(*key) == other
+    assert built_lib.sinv_gemm_select(200, 120, 64, 8, 0, 2, 1, 0) == default  # other batch
+    assert built_lib.sinv_gemm_force_config(n - 1) == 0
+    assert built_lib.sinv_gemm_select(*key) == n - 1
+    assert built_lib.sinv_gemm_force_config(-1) == 0
+    assert built_lib.sinv_gemm_tune_set(*key, n) != 0
+    built_lib.sinv_gemm_tune_clear()
+    assert built_lib.sinv_gemm_select(*key) == default
+    # untuned: small grids get the 32 x 32 tile, large ones the 64 x 64 tile
+    assert tiles[built_lib.sinv_gemm_select(512, 512, 512, 1, 0, 0, 0, 0)] == (32, 32)
+    assert tiles[built_lib.sinv_gemm_select(384, 384, 384, 256, 0, 0, 0, 0)] == (64, 64)
+
+
+def test_autotune_table_save_load(tmp_path, built_lib):
+    import json
+    from sinv_amd import autotune
+    table = {(64, 64, 64, 256, 0, 0, 1, 0): {"best": 1, "ms": {0: 0.03, 1: 0.02}}}
+    p = tmp_path / "t.json"
+    rows = [dict(key=list(k), best=v["best"], ms={str(c): t for c, t in v["ms"].items()})
+            for k, v in table.items()]
+    p.write_text(json.dumps(dict(device="x", entries=rows)))
+    assert autotune.load(str(p), install_now=False) == table

@@ -102,3 +102,64 @@ def test_zinv_nan_input_propagates(built_lib):
     X = core.zinv_batched(A, check=False).cpu().numpy()
     assert np.allclose(X[0], np.eye(8))
     assert np.isnan(X[1]).any() and np.isnan(X[2]).all()
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (65, 33, 17), (128, 96, 200), (130, 140, 40)])
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("T", "N"), ("N", "C"), ("C", "T")])
+def test_zgemm_every_tile_config(built_lib, M, N, K, opA, opB):
+    """Every tile configuration against numpy, and all of them bit-identical to
+    each other: they differ only in which workgroup computes which outputs, the
+    summation over k is the same chunked MFMA sequence in each."""
+    from sinv_amd import autotune, core
+    import torch
+    rng = np.random.default_rng(M + 7 * N + 13 * K)
+    batch = 2
+    A = _rand(rng, batch, *((M, K) if opA == "N" else (K, M)))
+    B = _rand(rng, batch, *((K, N) if opB == "N" else (N, K)))
+    C0 = _rand(rng, batch, M, N)
+    ref = 0.5 * (_op(A, opA) @ _op(B, opB)) - C0
+    outs = []
+    for cfg, tile in autotune.configs():
+        print(f"zgemm cfg={cfg} tile={tile} M={M} N={N} K={K} {opA}{opB}", flush=True)
+        built_lib.sinv_gemm_force_config(cfg)
+        try:
+            for prec in ("fp64", "mixed"):
+                C = core.zgemm_batched(A, B, opA=opA, opB=opB, alpha=0.5, beta=-1.0, C0=C0,
+                                       precision=prec)
+                got = C.cpu().numpy()
+                err = np.abs(got - ref).max() / np.abs(ref).max()
+                assert err < (1e-13 if prec == "fp64" else 1e-5), (cfg, prec, err)
+                if prec == "fp64":
+                    outs.append(C)
+        finally:
+            built_lib.sinv_gemm_force_config(-1)
+    for cfg, C in enumerate(outs[1:], 1):
+        assert torch.equal(C, outs[0]), cfg
+
+
+def test_autotune_installs_and_keeps_results(built_lib):
+    """Tuning a small batched RGF installs one config per GEMM shape (the one
+    that timed fastest) and leaves the sweep's results bit-identical."""
+    import torch
+    from sinv_amd import autotune, core
+    from sinv_amd.utils import configs
+    H = configs.hermitian_H(6, 96, seed=3)
+    E = configs.energies(4, seed=4) + 1e-2j
+    plan = core.RgfPlan(*H, shift=E, dscale=-1.0, oscale=-1.0)
+    autotune.clear()
+    plan.run()
+    torch.cuda.synchronize()
+    before = tuple(x.clone() for x in plan.outputs())
+    table = autotune.tune(plan.run, reps=1)
+    try:
+        assert table
+        for key, v in table.items():
+            assert autotune.select(key) == v["best"]
+            assert set(v["ms"]) == {c for c, _ in autotune.configs()}
+        plan.run()
+        torch.cuda.synchronize()
+        plan.check()
+        for a, b in zip(plan.outputs(), before):
+            assert torch.equal(a, b)
+    finally:
+        autotune.clear()
