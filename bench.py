@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the batch=1 FP64 sample")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the GEMM tile autotuner (untimed, before the timed steps)")
+    ap.add_argument("--save-tune", default=None, help="write the tuned GEMM table (JSON) here")
+    ap.add_argument("--load-tune", default=None,
+                    help="install this tuned GEMM table instead of tuning")
     ap.add_argument("--traffic-pass", action="store_true",
                     help="run one FP64 step and a 2 GiB calibration copy (neg kernel), untimed, "
                          "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs; prints no bench line")
@@ -145,10 +148,17 @@ def main():
     stream = torch.cuda.current_stream()
 
     if args.traffic_pass:
+        # counters see every dispatch of the process: tune in an earlier run
+        # (--save-tune) and load that table here, so only the step is counted
         plans = make_plans("fp64")
-        if not args.no_tune:  # count the same tile configurations the timed bench runs
+        if args.load_tune:
             from sinv_amd import autotune
-            autotune.tune(lambda: [p.run() for p in plans], reps=2)
+            autotune.load(args.load_tune)
+        elif not args.no_tune:
+            from sinv_amd import autotune
+            table = autotune.tune(lambda: [p.run() for p in plans], reps=2)
+            if args.save_tune:
+                autotune.save(table, args.save_tune)
         for p in plans:
             p.run()
         torch.cuda.synchronize()
@@ -190,11 +200,16 @@ def main():
     for prec in ("fp64", "mixed"):
         log(f"[bench] {wl['workload']}: batched {prec}: allocating plans")
         plans = make_plans(prec)
-        if not args.no_tune:
+        if args.load_tune:
+            from sinv_amd import autotune
+            autotune.load(args.load_tune)
+        elif not args.no_tune:
             # GEMM tile configs per shape, timed on scratch operands (outside the timed region)
             from sinv_amd import autotune
             t0 = time.perf_counter()
             table = autotune.tune(lambda: [p.run() for p in plans], reps=2)
+            if args.save_tune:
+                autotune.save(table, args.save_tune + f".{prec}")
             tuned[prec] = dict(shapes=len(table), seconds=round(time.perf_counter() - t0, 2),
                                chosen={str(c): sum(1 for v in table.values() if v["best"] == c)
                                        for c, _ in autotune.configs()})

@@ -46,12 +46,14 @@ def stats_table(d):
     f = _one(os.path.join(d, "**", "*kernel_stats.csv")) or _one(os.path.join(d, "*kernel_stats.csv"))
     if not f:
         return []
-    rows = list(csv.DictReader(open(f)))
-    out = []
-    for r in rows:
-        out.append((_short(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]) / 1e6,
-                    float(r["AverageNs"]) / 1e3, float(r["Percentage"])))
-    return out
+    agg = {}
+    for r in csv.DictReader(open(f)):
+        a = agg.setdefault(_short(r["Name"]), [0, 0.0, 0.0])
+        a[0] += int(r["Calls"])
+        a[1] += float(r["TotalDurationNs"]) / 1e6
+        a[2] += float(r["Percentage"])
+    return [(k, n, t, t / n * 1e3, pct) for k, (n, t, pct) in
+            sorted(agg.items(), key=lambda kv: -kv[1][1])]
 
 
 def _ranges(d, name):
