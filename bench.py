@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--energies", type=int, default=256)
     ap.add_argument("--workload", default="energies",
-                    choices=["energies", "dominant", "large", "adjoint"])
+                    choices=["energies", "dominant", "large", "adjoint", "convert"])
     ap.add_argument("--chunk", type=int, default=64, help="energies per adjoint launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the batch=1 FP64 sample")
@@ -105,6 +105,8 @@ def main():
     torch.cuda.synchronize()
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "convert":
+        return bench_convert(args)
 
     # ------------------------------------------------------------ workload
     eta = 1e-3
@@ -494,6 +496,59 @@ def bench_adjoint(args):
         "kernel_time": {k: dict(ms=round(v[0], 3), launches=v[1]) for k, v in st.items() if v[1]},
     }
     print(json.dumps(res), flush=True)
+    return 0
+
+
+
+def bench_convert(args):
+    """Device dense <-> block-tridiagonal converters on a 32768 x 32768 complex128
+    matrix with 512 x 512 blocks (17.2 GB dense, 64 blocks): gather of the
+    3N - 2 band blocks, and the dense result (band blocks + zeros).  HBM-bound:
+    algorithmic bytes = band read + band written (gather), band read + n^2
+    written (dense); reported against 8 TB/s."""
+    import torch
+    from sinv_amd import core
+    n, b = 32768, 512
+    N = n // b
+    dev = core.device()
+    A = torch.empty((n, n), dtype=torch.complex128, device=dev)
+    A.real.normal_()
+    band = (3 * N - 2) * b * b * 16
+    res = {}
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    blocks = core.dense_to_blocks(A, b)
+    ms = timed(lambda: core.dense_to_blocks(A, b), max(args.steps, 1) * 10)
+    res["dense_to_blocks"] = dict(ms=round(ms, 4), bytes=2 * band, gbps=round(2 * band / ms / 1e6, 1))
+    del A
+    torch.cuda.empty_cache()
+    ms = timed(lambda: core.blocks_to_dense(*blocks), max(args.steps, 1) * 3)
+    out_bytes = band + n * n * 16
+    res["blocks_to_dense"] = dict(ms=round(ms, 4), bytes=out_bytes, gbps=round(out_bytes / ms / 1e6, 1))
+    log(f"[bench] convert: {res}")
+    head = res["blocks_to_dense"]
+    print(json.dumps({
+        "metric": "GB/s: device dense<->block-tridiagonal converters (matrices_utils.py:115-189)",
+        "value": head["gbps"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
+        "config": dict(workload="dense_block_convert", n=n, blocksize=b, blocks=N),
+        "roofline": dict(bound="hbm", achieved=head["gbps"], peak=8000.0, unit="GB/s",
+                         frac=round(head["gbps"] / 8000.0, 4), traffic=None,
+                         kernel="blocks_to_dense_kernel"),
+        "cpu_baseline": None,
+        "kernels": res}), flush=True)
     return 0
 
 

@@ -239,6 +239,29 @@ int sinv_zgemm_batched(int M, int N, int K, int opA, int opB, sinv_c128 alpha,
                        const void* shift, int batch, int precision, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Dense <-> block-tridiagonal conversion on the device.
+ * Replaces matrices_utils.py:115-151 (convertDenseToBlkTridiag) and
+ * matrices_utils.py:155-189 (convertBlkTridiagToDense) for matrices resident
+ * in HBM; complex128, row-major, `batch` matrices at stride strideA.
+ *  - sinv_dense_to_blocks: n must be a multiple of b (the reference's ragged
+ *    last block raises numpy's broadcast error); writes diag (batch,N,b,b),
+ *    upper/lower (batch,N-1,b,b), contiguous.
+ *  - sinv_blocks_to_dense: writes the whole (N b) x (N b) matrix: band blocks
+ *    from the stacked arrays, zeros elsewhere.
+ *  - sinv_scatter_blocks: copies `count` b x b blocks (stride strideS between
+ *    batch entries, contiguous within) to the block positions (rows[k],
+ *    cols[k]) given in DEVICE int arrays; positions outside the nblocks x
+ *    nblocks grid are skipped.
+ * ------------------------------------------------------------------------ */
+int sinv_dense_to_blocks(const void* A, int n, int lda, long long strideA, int b, int batch,
+                         void* diag, void* upper, void* lower, void* stream);
+int sinv_blocks_to_dense(const void* diag, const void* upper, const void* lower, int nblocks,
+                         int b, int batch, void* A, int lda, long long strideA, void* stream);
+int sinv_scatter_blocks(const void* blocks, long long strideS, int count, const int* rows,
+                        const int* cols, int nblocks, int b, int batch, void* A, int lda,
+                        long long strideA, void* stream);
+
+/* ------------------------------------------------------------------------
  * GEMM tile configurations and tuned shapes (no counterpart in the reference:
  * numpy's BLAS picks its own blocking).  Every complex GEMM the library issues
  * (sinv_zgemm_batched and the ones inside the RGF, BCR, PSR, adjoint and

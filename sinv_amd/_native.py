@@ -128,6 +128,12 @@ def lib():
     L.sinv_prof_end.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_ll),
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                 _i]
+    L.sinv_dense_to_blocks.restype = _i
+    L.sinv_dense_to_blocks.argtypes = [_vp, _i, _i, _ll, _i, _i, _vp, _vp, _vp, _vp]
+    L.sinv_blocks_to_dense.restype = _i
+    L.sinv_blocks_to_dense.argtypes = [_vp, _vp, _vp, _i, _i, _i, _vp, _i, _ll, _vp]
+    L.sinv_scatter_blocks.restype = _i
+    L.sinv_scatter_blocks.argtypes = [_vp, _ll, _i, _vp, _vp, _i, _i, _i, _vp, _i, _ll, _vp]
     L.sinv_gemm_config_count.restype = _i
     L.sinv_gemm_config_count.argtypes = []
     L.sinv_gemm_config_tile.restype = _i

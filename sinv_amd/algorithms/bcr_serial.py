@@ -8,7 +8,9 @@ cropped to ``(n // blocksize) * blocksize`` (bcr_serial.py:39,60).  Only the
 block-tridiagonal part of A is read, as in the reference.  The reduction and
 production run on the GPU (libsinv_hip ``sinv_bcr_batched``).
 
-``bcr_blocks`` is the stacked-block form (numpy in, numpy out).
+``bcr_blocks`` is the stacked-block form (numpy in, numpy out).  A matrix
+already in HBM (a torch device tensor) stays there: its blocks are gathered,
+and the dense result assembled, by the device converters.
 """
 
 import numpy as np
@@ -28,7 +30,24 @@ def _blocks_of(A, b, N):
     return d, u, l
 
 
+def _bcr_serial_device(A, b):
+    # A resident in HBM (torch device tensor): gather, solve and assemble the
+    # dense result on the device; returns a device tensor
+    import torch
+    N = A.shape[0] // b
+    Ad = A[:N * b, :N * b].to(torch.complex128)
+    d, u, l = core.dense_to_blocks(Ad, b)
+    Gd, Gu, Gl, Gf = core.bcr_batched(d, u, l, return_fill=True)
+    G = core.blocks_to_dense(Gd, Gu, Gl)
+    core.scatter_blocks(G, Gf, core.bcr_fill_positions(N))
+    G = G[0]
+    return G.real.to(A.dtype) if not A.is_complex() else G.to(A.dtype)
+
+
 def bcr_serial(A, blocksize):
+    from ..utils.matu import _on_device
+    if _on_device(A):
+        return _bcr_serial_device(A, int(blocksize))
     A = np.asarray(A)
     b = int(blocksize)
     N = A.shape[0] // b

@@ -523,6 +523,70 @@ def rgf_energies(H_diag, H_upper, H_lower, energies, eta, **kw):
     return rgf_batched(H_diag, H_upper, H_lower, shift=e, dscale=-1.0, oscale=-1.0, **kw)
 
 
+def dense_to_blocks(A, blocksize):
+    """Stacked tridiagonal blocks (diag (B,N,b,b), upper/lower (B,N-1,b,b)) of a
+    dense (n, n) or (B, n, n) complex128 matrix, gathered on the device
+    (matrices_utils.py:115-151).  n must be a multiple of blocksize."""
+    A = to_dev(A)
+    if A.dim() == 2:
+        A = A[None]
+    A = A.contiguous()
+    B, n = A.shape[0], A.shape[-1]
+    b = int(blocksize)
+    if A.shape[-2] != n or b <= 0 or n % b != 0:
+        raise ValueError(f"dense_to_blocks: square matrices with n a multiple of the block size "
+                         f"(n={n}, b={b})")
+    N = n // b
+    d = torch.empty((B, N, b, b), dtype=_c128, device=A.device)
+    u = torch.empty((B, max(N - 1, 0), b, b), dtype=_c128, device=A.device)
+    l = torch.empty((B, max(N - 1, 0), b, b), dtype=_c128, device=A.device)
+    nat.check(nat.lib().sinv_dense_to_blocks(_p(A), n, n, n * n, b, B, _p(d),
+                                             _p(u) if N > 1 else None, _p(l) if N > 1 else None,
+                                             _stream()), "dense_to_blocks")
+    return d, u, l
+
+
+def blocks_to_dense(diag, upper, lower):
+    """Dense (B, N b, N b) matrices from stacked tridiagonal blocks, zeros
+    elsewhere, written on the device (matrices_utils.py:155-189)."""
+    d, _, _ = _batched(to_dev(diag), 3)
+    if d.dim() == 3:
+        d = d[None]
+    B, N, b = d.shape[0], d.shape[1], d.shape[-1]
+    d = d.contiguous()
+    if N > 1:
+        u = to_dev(upper).reshape(B, N - 1, b, b).contiguous()
+        l = to_dev(lower).reshape(B, N - 1, b, b).contiguous()
+    else:
+        u = l = None
+    n = N * b
+    A = torch.empty((B, n, n), dtype=_c128, device=d.device)
+    nat.check(nat.lib().sinv_blocks_to_dense(_p(d), _p(u), _p(l), N, b, B, _p(A), n, n * n,
+                                             _stream()), "blocks_to_dense")
+    return A
+
+
+def scatter_blocks(A, blocks, positions):
+    """Copy blocks (B, count, b, b) into the dense (B, n, n) A at the block
+    positions [(row, col), ...] (in place, on the device)."""
+    blocks = to_dev(blocks)
+    if blocks.dim() == 3:
+        blocks = blocks[None]
+    blocks = blocks.contiguous()
+    B, count, b = blocks.shape[0], blocks.shape[1], blocks.shape[-1]
+    if count == 0:
+        return A
+    pos = torch.as_tensor(np.asarray(positions, dtype=np.int32).reshape(count, 2),
+                          device=A.device)
+    rows = pos[:, 0].contiguous()
+    cols = pos[:, 1].contiguous()
+    n = A.shape[-1]
+    nat.check(nat.lib().sinv_scatter_blocks(_p(blocks), count * b * b, count, _p(rows), _p(cols),
+                                            n // b, b, B, _p(A), n, n * n, _stream()),
+              "scatter_blocks")
+    return A
+
+
 def zinv_batched(A, check=True):
     """In-place-style batched inverse of (..., n, n) complex128 blocks; returns a new tensor."""
     X = to_dev(A).clone()

@@ -35,7 +35,21 @@ def transformToSymmetric(A):
     return A + A.T
 
 
+def _on_device(x):
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        return False
+    return isinstance(x, torch.Tensor) and x.is_cuda
+
+
 def convertDenseToBlkTridiag(A, blocksize):
+    # a matrix already in HBM (torch device tensor) is gathered on the device;
+    # returns device tensors (N,b,b), (N-1,b,b), (N-1,b,b)
+    if _on_device(A):
+        from .. import core
+        d, u, l = core.dense_to_blocks(A, blocksize)
+        return d[0], u[0], l[0]
     # N = ceil(n / b) as in the reference; a ragged last block raises numpy's
     # broadcast ValueError there and here alike
     N = int(np.ceil(A.shape[0] / blocksize))
@@ -52,6 +66,9 @@ def convertDenseToBlkTridiag(A, blocksize):
 
 
 def convertBlkTridiagToDense(A_diagblk, A_upperblk, A_lowerblk):
+    if _on_device(A_diagblk):
+        from .. import core
+        return core.blocks_to_dense(A_diagblk, A_upperblk, A_lowerblk)[0]
     N, b = A_diagblk.shape[0], A_diagblk.shape[1]
     A = np.zeros((N * b, N * b), dtype=A_diagblk.dtype)
     for i in range(N):

@@ -102,3 +102,17 @@ def test_autotune_table_save_load(tmp_path, built_lib):
             for k, v in table.items()]
     p.write_text(json.dumps(dict(device="x", entries=rows)))
     assert autotune.load(str(p), install_now=False) == table
+
+
+def test_converters_reject_bad_arguments_on_the_host(built_lib):
+    from sinv_amd import _native as nat
+    buf = ctypes.c_void_p(16)  # never dereferenced: validation fails first
+    assert built_lib.sinv_dense_to_blocks(buf, 100, 100, 10000, 30, 1, buf, buf, buf, None) \
+        == nat.SINV_E_INVAL  # 100 is not a multiple of 30
+    assert b"multiple" in built_lib.sinv_last_error()
+    assert built_lib.sinv_dense_to_blocks(None, 90, 90, 8100, 30, 1, buf, buf, buf, None) \
+        == nat.SINV_E_INVAL
+    assert built_lib.sinv_blocks_to_dense(buf, None, None, 3, 8, 1, buf, 24, 576, None) \
+        == nat.SINV_E_INVAL  # upper/lower missing for 3 blocks
+    assert built_lib.sinv_scatter_blocks(buf, 0, 1, buf, buf, 2, 8, 1, buf, 8, 256, None) \
+        == nat.SINV_E_INVAL  # lda smaller than the matrix
