@@ -212,6 +212,36 @@ size_t sinv_psr_workspace_bytes(const sinv_psr_desc* d);
 int sinv_psr_batched(const sinv_psr_desc* d, void* work, size_t work_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
+ * PDIV (P-Division) selected inversion with npart partitions on one device.
+ * Replaces pdiv_localmap.py:27-90 (pdiv_localmap on every rank) with the
+ * partitioning of pdiv_utils.py:84-170: npart must be a power of 2 and at
+ * most nblocks; partition 0 takes the remainder N - m (npart - 1),
+ * m = N // npart.  Each partition is inverted densely (invert_partition,
+ * pdiv_localmap.py:94-115) and its selected blocks produced from that inverse
+ * (produce_matrix_elements :1194-1241), batched over partitions and
+ * instances; the coupling between partitions (the reference's map updates)
+ * is a Woodbury correction from two sweeps of b x b boundary blocks.
+ * Outputs: the block-tridiagonal part of A^-1, diag (batch,N,b,b),
+ * upper/lower (batch,N-1,b,b), contiguous.  Strides of 0 share the input.
+ * ------------------------------------------------------------------------ */
+typedef struct sinv_pdiv_desc {
+  int nblocks;
+  int blocksize;
+  int batch;
+  int npart;
+  const void* diag;  long long diag_stride;
+  const void* upper; long long upper_stride;
+  const void* lower; long long lower_stride;
+  void* g_diag;
+  void* g_upper;
+  void* g_lower;
+  unsigned long long* info;
+} sinv_pdiv_desc;
+
+size_t sinv_pdiv_workspace_bytes(const sinv_pdiv_desc* d);
+int sinv_pdiv_batched(const sinv_pdiv_desc* d, void* work, size_t work_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
  * Batched in-place complex128 block inverse (kernel K1).
  * Replaces the numpy.linalg.inv calls of the hot path (rgf.py:44,48;
  * bcr_serial.py:121,148,249,302,379).  Blocks with n <= 64 use Gauss-Jordan

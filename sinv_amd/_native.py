@@ -76,6 +76,17 @@ class PsrDesc(ctypes.Structure):
     ]
 
 
+class PdivDesc(ctypes.Structure):
+    _fields_ = [
+        ("nblocks", _i), ("blocksize", _i), ("batch", _i), ("npart", _i),
+        ("diag", _vp), ("diag_stride", _ll),
+        ("upper", _vp), ("upper_stride", _ll),
+        ("lower", _vp), ("lower_stride", _ll),
+        ("g_diag", _vp), ("g_upper", _vp), ("g_lower", _vp),
+        ("info", ctypes.POINTER(ctypes.c_ulonglong)),
+    ]
+
+
 class AdjDesc(ctypes.Structure):
     _fields_ = [
         ("nblocks", _i), ("blocksize", _i), ("bsizes", ctypes.POINTER(_i)), ("batch", _i),
@@ -128,6 +139,10 @@ def lib():
     L.sinv_prof_end.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_ll),
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                 _i]
+    L.sinv_pdiv_workspace_bytes.restype = ctypes.c_size_t
+    L.sinv_pdiv_workspace_bytes.argtypes = [ctypes.POINTER(PdivDesc)]
+    L.sinv_pdiv_batched.restype = _i
+    L.sinv_pdiv_batched.argtypes = [ctypes.POINTER(PdivDesc), _vp, ctypes.c_size_t, _vp]
     L.sinv_dense_to_blocks.restype = _i
     L.sinv_dense_to_blocks.argtypes = [_vp, _i, _i, _ll, _i, _i, _vp, _vp, _vp, _vp]
     L.sinv_blocks_to_dense.restype = _i

@@ -1,12 +1,15 @@
 """Namespace mirroring ``sinv.algorithms`` (src/sinv/algorithms/__init__.py:3-9):
 ``rgf`` (``rgf.rgf``), ``bcr_s`` (``bcr_s.bcr_serial``), and the MPI variants'
 partition-local work on one GPU: ``rgf2s`` (``rgf2s.rgf2sided``) and
-``psr_s`` (``psr_s.psr_seqsolve``), with partitions in place of ranks.
-BCR-parallel and PDIV are not provided."""
+``psr_s`` (``psr_s.psr_seqsolve``) and ``pdiv_lm`` / ``pdiv_u`` (PDIV: one
+call ``pdiv_lm.pdiv`` returns every rank's ``pdiv_localmap`` output), with
+partitions in place of ranks.  BCR-parallel is not provided."""
 
 from . import rgf as rgf
 from . import bcr_serial as bcr_s
 from . import rgf2sided as rgf2s
 from . import psr_seqsolve as psr_s
+from . import pdiv as pdiv_lm
+from . import pdiv as pdiv_u
 
-__all__ = ["rgf", "bcr_s", "rgf2s", "psr_s"]
+__all__ = ["rgf", "bcr_s", "rgf2s", "psr_s", "pdiv_lm", "pdiv_u"]

@@ -500,6 +500,45 @@ def psr_batched(diag, upper, lower, npart, *, shift=None, dscale=1.0, oscale=1.0
     return Gd, Gu, Gl
 
 
+def pdiv_batched(diag, upper, lower, npart, *, check=True):
+    """PDIV selected inversion (pdiv_localmap.py:27-90) with ``npart`` partitions
+    on one GPU: dense partition inverses and their selected-block production
+    batched over partitions, coupled by a Woodbury correction.  npart must be a
+    power of 2 and at most nblocks (pdiv_utils.check_multiprocessing /
+    check_input).  Same instance convention and outputs as ``rgf_batched``."""
+    npart = int(npart)
+    d, Bd, sd = _batched(to_dev(diag), 3)
+    N, b = d.shape[-3], d.shape[-1]
+    if npart < 1 or npart & (npart - 1):
+        raise ValueError("The number of processes must be a power of 2.")
+    if N < npart:
+        raise ValueError("The number of blocks must be greater than the number of processes.")
+    if N > 1:
+        u, Bu, su = _batched(to_dev(upper), 3)
+        l, Bl, sl = _batched(to_dev(lower), 3)
+    else:
+        u = l = None
+        Bu = Bl = None
+        su = sl = 0
+    B = _resolve_batch(Bd, Bu, Bl, None)
+    dev = device()
+    Gd = torch.empty((B, N, b, b), dtype=_c128, device=dev)
+    Gu = torch.empty((B, max(N - 1, 0), b, b), dtype=_c128, device=dev)
+    Gl = torch.empty((B, max(N - 1, 0), b, b), dtype=_c128, device=dev)
+    info = torch.empty(1, dtype=torch.int64, device=dev)
+    desc = nat.PdivDesc(
+        nblocks=N, blocksize=b, batch=B, npart=npart, diag=_p(d), diag_stride=sd,
+        upper=_p(u), upper_stride=su, lower=_p(l), lower_stride=sl, g_diag=_p(Gd),
+        g_upper=_p(Gu) if N > 1 else None, g_lower=_p(Gl) if N > 1 else None,
+        info=_info_ptr(info))
+    L = nat.lib()
+    ws = _workspace(L.sinv_pdiv_workspace_bytes(ctypes.byref(desc)))
+    nat.check(L.sinv_pdiv_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()), "pdiv")
+    if check:
+        nat.check_info(info)
+    return Gd, Gu, Gl
+
+
 def bcr_fill_positions(N):
     """(row, col) block positions of the blocks bcr_batched(return_fill=True) returns."""
     out = []

@@ -1,0 +1,96 @@
+"""The PDIV oracle (oracle/pdiv_ref.py, a restatement of pdiv_localmap.py with
+the ranks simulated) pinned to the dense inverse, the reference's own
+acceptance test (pdiv_test.py:70-115): at 1e-12 on diagonally dominant
+versions of the reference's test matrices, at the reference's allclose on the
+raw ones (their partitions can be ill-conditioned: PDIV inverts them densely),
+and against planted mistakes."""
+
+import numpy as np
+import pytest
+
+from oracle import dense, pdiv_ref
+from sinv_amd.algorithms import pdiv as pdiv_dropin
+from sinv_amd.utils import matu
+
+SHAPES = [(2, 1), (4, 2), (6, 3), (3, 1), (6, 2), (9, 3), (128, 8), (128, 16), (128, 32)]
+
+
+def _cases():
+    for n, b in SHAPES:
+        for P in (1, 2, 4, 8):
+            if P <= n // b:
+                yield n, b, P
+
+
+def _matrix(n, b, cplx, sym, shift):
+    A = matu.generateBandedDiagonalMatrix(n, np.ceil(b / 2), cplx, sym, 63)
+    return A + shift * b * np.eye(n) if shift else A
+
+
+@pytest.mark.parametrize("n,b,P", list(_cases()))
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("sym", [False, True])
+def test_pdiv_oracle_vs_dense(n, b, P, cplx, sym):
+    A = _matrix(n, b, cplx, sym, 3)
+    G = pdiv_ref.stacked(pdiv_ref.pdiv_blocks(A, b, P))
+    assert dense.rel_err(G, dense.select(np.linalg.inv(A), b)) < 1e-12
+    Araw = _matrix(n, b, cplx, sym, 0)
+    out = pdiv_ref.pdiv_blocks(Araw, b, P)
+    Rd, Ru, Rl = dense.select(np.linalg.inv(Araw), b)
+    Gd, Gu, Gl = pdiv_ref.stacked(out)
+    assert np.allclose(Gd, Rd) and np.allclose(Gu, Ru) and np.allclose(Gl, Rl)
+
+
+def test_pdiv_oracle_per_rank_layout():
+    """Rank r's lists hold its partition's blocks; the last upper/lower entries
+    are the bridges to rank r + 1, zero on the last rank (pdiv_test.py:92-109)."""
+    A = _matrix(128, 16, True, False, 3)
+    out = pdiv_ref.pdiv_blocks(A, 16, 4)
+    X = np.linalg.inv(A)
+    starts, sizes = pdiv_ref.divide_matrix(A, 4, 16)
+    for r, (D, U, L) in enumerate(out):
+        assert len(D) == len(U) == len(L) == sizes[r]
+        e = starts[r] + sizes[r] - 1
+        blk = lambda i, j: X[i * 16:(i + 1) * 16, j * 16:(j + 1) * 16]
+        if r < 3:
+            assert np.allclose(U[-1], blk(e, e + 1), atol=1e-14)
+            assert np.allclose(L[-1], blk(e + 1, e), atol=1e-14)
+        else:
+            assert not np.any(U[-1]) and not np.any(L[-1])
+
+
+def test_pdiv_oracle_planted_mistakes(monkeypatch):
+    A = _matrix(64, 8, True, False, 3)
+    R = dense.select(np.linalg.inv(A), 8)
+    ok = pdiv_ref.stacked(pdiv_ref.pdiv_blocks(A, 8, 4))
+    assert dense.rel_err(ok, R) < 1e-12
+    # 1. bridges swapped (upper used where the lower belongs)
+    orig = pdiv_ref.partition_subdomain
+    monkeypatch.setattr(pdiv_ref, "partition_subdomain",
+                        lambda *a: (lambda K, Bu, Bl: (K, Bl, Bu))(*orig(*a)))
+    assert dense.rel_err(pdiv_ref.stacked(pdiv_ref.pdiv_blocks(A, 8, 4)), R) > 1e-6
+    monkeypatch.setattr(pdiv_ref, "partition_subdomain", orig)
+    # 2. produce_matrix_elements without its first update term
+    orig_elem = pdiv_ref._elem
+
+    def elem_missing_term(r, c, K, M, b):
+        M = list(M)
+        M[8] = np.zeros_like(M[8])
+        return orig_elem(r, c, K, M, b)
+    monkeypatch.setattr(pdiv_ref, "_elem", elem_missing_term)
+    assert dense.rel_err(pdiv_ref.stacked(pdiv_ref.pdiv_blocks(A, 8, 4)), R) > 1e-6
+
+
+def test_dropin_helpers_match_oracle_and_checks():
+    A = _matrix(96, 8, False, False, 3)
+    assert pdiv_dropin.divide_matrix(A, 4, 8) == pdiv_ref.divide_matrix(A, 4, 8)
+    st, sz = pdiv_dropin.divide_matrix(A, 4, 8)
+    K1, U1, L1 = pdiv_dropin.partition_subdomain(A, st, sz, 8)
+    K2, U2, L2 = pdiv_ref.partition_subdomain(A, st, sz, 8)
+    assert all(np.array_equal(x, y) for x, y in zip(K1 + U1 + L1, K2 + U2 + L2))
+    with pytest.raises(ValueError, match="power of 2"):
+        pdiv_dropin.check_multiprocessing(3)
+    with pytest.raises(ValueError, match="multiple"):
+        pdiv_dropin.check_input(np.zeros((10, 10)), 3, 2)
+    with pytest.raises(ValueError, match="greater than the number of processes"):
+        pdiv_dropin.check_input(np.zeros((12, 12)), 4, 4)
