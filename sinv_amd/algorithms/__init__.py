@@ -3,13 +3,16 @@
 partition-local work on one GPU: ``rgf2s`` (``rgf2s.rgf2sided``) and
 ``psr_s`` (``psr_s.psr_seqsolve``) and ``pdiv_lm`` / ``pdiv_u`` (PDIV: one
 call ``pdiv_lm.pdiv`` returns every rank's ``pdiv_localmap`` output), with
-partitions in place of ranks.  BCR-parallel is not provided."""
+partitions in place of ranks, and ``bcr_p`` (``bcr_p.bcr_parallel``: every
+rank's rows of a level are one batched launch, rank 0's gathered result is
+returned)."""
 
 from . import rgf as rgf
 from . import bcr_serial as bcr_s
+from . import bcr_parallel as bcr_p
 from . import rgf2sided as rgf2s
 from . import psr_seqsolve as psr_s
 from . import pdiv as pdiv_lm
 from . import pdiv as pdiv_u
 
-__all__ = ["rgf", "bcr_s", "rgf2s", "psr_s", "pdiv_lm", "pdiv_u"]
+__all__ = ["rgf", "bcr_s", "bcr_p", "rgf2s", "psr_s", "pdiv_lm", "pdiv_u"]

@@ -140,80 +140,87 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
   const cplx* A = g.A + tm.z_in * g.sA_in + tm.z_out * g.sA_out;
   const cplx* B = g.B + tm.z_in * g.sB_in + tm.z_out * g.sB_out;
 
-  // op(A) tile is BM x BK, op(B) tile is BK x BN; each thread loads groups of
-  // VA (VB) elements consecutive in memory (along k for A stored M x K, along m
-  // for A stored K x M; along n for B stored K x N, along k for B stored N x K)
+  // op(A) tile is BM x BK, op(B) tile is BK x BN, staged k-major in LDS.  Each
+  // thread moves EA (EB) elements, 16 B each.  An operand stored contiguous
+  // along k (A as M x K, B as N x K) is read in runs of VA (VB) consecutive k
+  // per thread: a run lands in VA different LDS rows, and the padded row
+  // stride spreads the 16 lanes of a quarter-wave over all banks.  An operand
+  // contiguous along m or n is read lane-consecutively along that dimension:
+  // each wave reads whole rows (coalesced) and writes them to one LDS row
+  // (bank-conflict free); runs along n would put 16 lanes on 4 bank groups.
+  constexpr int EA = GA * VA, EB = GB * VB;
   const bool a_kcontig = (g.opA == OP_N);
   const bool b_ncontig = (g.opB == OP_N);
   const double a_conj = (g.opA == OP_C) ? -1.0 : 1.0;
   const double b_conj = (g.opB == OP_C) ? -1.0 : 1.0;
 
-  int a_r[GA], a_c[GA];  // (m, k) of each group's first element, tile coordinates
+  int am[EA], ak[EA];  // tile coordinates (m, k) of each element this thread moves
 #pragma unroll
-  for (int q = 0; q < GA; ++q) {
-    const int L = tid + q * NT;
-    if (a_kcontig) { a_r[q] = L / (BK / VA); a_c[q] = (L % (BK / VA)) * VA; }
-    else           { a_c[q] = L / (BMt / VA); a_r[q] = (L % (BMt / VA)) * VA; }
+  for (int j = 0; j < EA; ++j) {
+    if (a_kcontig) {
+      const int L = tid + (j / VA) * NT;
+      am[j] = L / (BK / VA);
+      ak[j] = (L % (BK / VA)) * VA + j % VA;
+    } else {
+      const int E = tid + j * NT;
+      ak[j] = E / BMt;
+      am[j] = E % BMt;
+    }
   }
-  int b_r[GB], b_c[GB];  // (k, n)
+  int bk[EB], bn[EB];  // (k, n)
 #pragma unroll
-  for (int q = 0; q < GB; ++q) {
-    const int L = tid + q * NT;
-    if (b_ncontig) { b_r[q] = L / (BNt / VB); b_c[q] = (L % (BNt / VB)) * VB; }
-    else           { b_c[q] = L / (BK / VB); b_r[q] = (L % (BK / VB)) * VB; }
+  for (int j = 0; j < EB; ++j) {
+    if (b_ncontig) {
+      const int E = tid + j * NT;
+      bk[j] = E / BNt;
+      bn[j] = E % BNt;
+    } else {
+      const int L = tid + (j / VB) * NT;
+      bn[j] = L / (BK / VB);
+      bk[j] = (L % (BK / VB)) * VB + j % VB;
+    }
   }
 
-  cplx ra[GA][VA], rb[GB][VB];
-  bool oka[GA][VA], okb[GB][VB];
+  struct Regs {
+    cplx ra[EA], rb[EB];
+    bool oka[EA], okb[EB];
+  };
+  Regs R0;
 
   // Branch-free tile loads: an out-of-range element is read from (0, 0), which
   // always exists, and zeroed when staged into LDS.  All loads of a chunk are
   // then in flight together instead of each behind its own branch.
-  auto load_tiles = [&](int k0) {
+  auto load_tiles = [&](Regs& X, int k0) {
 #pragma unroll
-    for (int q = 0; q < GA; ++q)
+    for (int j = 0; j < EA; ++j) {
+      const int m = tm.m0 + am[j], k = k0 + ak[j];
+      X.oka[j] = (m < g.M) && (k < g.K);
+      const long long mm = X.oka[j] ? m : 0, kk = X.oka[j] ? k : 0;
+      X.ra[j] = a_kcontig ? ld_c(A + mm * g.lda + kk) : ld_c(A + kk * g.lda + mm);
+    }
 #pragma unroll
-      for (int e = 0; e < VA; ++e) {
-        const int m = tm.m0 + a_r[q] + (a_kcontig ? 0 : e);
-        const int k = k0 + a_c[q] + (a_kcontig ? e : 0);
-        oka[q][e] = (m < g.M) && (k < g.K);
-        const long long mm = oka[q][e] ? m : 0, kk = oka[q][e] ? k : 0;
-        ra[q][e] = a_kcontig ? ld_c(A + mm * g.lda + kk) : ld_c(A + kk * g.lda + mm);
-      }
-#pragma unroll
-    for (int q = 0; q < GB; ++q)
-#pragma unroll
-      for (int e = 0; e < VB; ++e) {
-        const int k = k0 + b_r[q] + (b_ncontig ? 0 : e);
-        const int n = tm.n0 + b_c[q] + (b_ncontig ? e : 0);
-        okb[q][e] = (k < g.K) && (n < g.N);
-        const long long kk = okb[q][e] ? k : 0, nn = okb[q][e] ? n : 0;
-        rb[q][e] = b_ncontig ? ld_c(B + kk * g.ldb + nn) : ld_c(B + nn * g.ldb + kk);
-      }
+    for (int j = 0; j < EB; ++j) {
+      const int k = k0 + bk[j], n = tm.n0 + bn[j];
+      X.okb[j] = (k < g.K) && (n < g.N);
+      const long long kk = X.okb[j] ? k : 0, nn = X.okb[j] ? n : 0;
+      X.rb[j] = b_ncontig ? ld_c(B + kk * g.ldb + nn) : ld_c(B + nn * g.ldb + kk);
+    }
   };
-  auto store_tiles = [&](int buf) {
+  auto store_tiles = [&](const Regs& X, int buf) {
     auto& As = As2[buf];
     auto& Bs = Bs2[buf];
 #pragma unroll
-    for (int q = 0; q < GA; ++q)
+    for (int j = 0; j < EA; ++j) {
+      const double re = X.oka[j] ? X.ra[j].re : 0.0;
+      const double im = X.oka[j] ? X.ra[j].im * a_conj : 0.0;
+      As[ak[j]][am[j]] = CR<R>{(R)re, (R)im};
+    }
 #pragma unroll
-      for (int e = 0; e < VA; ++e) {
-        const int m = a_r[q] + (a_kcontig ? 0 : e);
-        const int k = a_c[q] + (a_kcontig ? e : 0);
-        const double re = oka[q][e] ? ra[q][e].re : 0.0;
-        const double im = oka[q][e] ? ra[q][e].im * a_conj : 0.0;
-        As[k][m] = CR<R>{(R)re, (R)im};
-      }
-#pragma unroll
-    for (int q = 0; q < GB; ++q)
-#pragma unroll
-      for (int e = 0; e < VB; ++e) {
-        const int k = b_r[q] + (b_ncontig ? 0 : e);
-        const int n = b_c[q] + (b_ncontig ? e : 0);
-        const double re = okb[q][e] ? rb[q][e].re : 0.0;
-        const double im = okb[q][e] ? rb[q][e].im * b_conj : 0.0;
-        Bs[k][n] = CR<R>{(R)re, (R)im};
-      }
+    for (int j = 0; j < EB; ++j) {
+      const double re = X.okb[j] ? X.rb[j].re : 0.0;
+      const double im = X.okb[j] ? X.rb[j].im * b_conj : 0.0;
+      Bs[bk[j]][bn[j]] = CR<R>{(R)re, (R)im};
+    }
   };
 
   using V = typename Acc<R>::V;
@@ -284,14 +291,14 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
   };
 
   const int ktiles = (g.K + BK - 1) / BK;
-  load_tiles(0);
-  store_tiles(0);
+  load_tiles(R0, 0);
+  store_tiles(R0, 0);
   __syncthreads();
   for (int kt = 0; kt < ktiles; ++kt) {
     const bool more = kt + 1 < ktiles;
-    if (more) load_tiles((kt + 1) * BK);
+    if (more) load_tiles(R0, (kt + 1) * BK);
     compute(kt & 1);
-    if (more) store_tiles((kt + 1) & 1);
+    if (more) store_tiles(R0, (kt + 1) & 1);
     __syncthreads();
   }
 

@@ -78,3 +78,11 @@ def test_product_path_has_no_oracle_import():
     for f in root.rglob("*.py"):
         txt = f.read_text()
         assert "import oracle" not in txt and "from oracle" not in txt, f
+
+
+def test_bcr_parallel_partitioning_helpers():
+    """bcr_utils.py:133-184: equal strides, the last partition takes the rest."""
+    from sinv_amd.algorithms import bcr_p
+    assert bcr_p.divide_matrix(16, 4) == ([0, 4, 8, 12], [4, 4, 4, 4])
+    assert bcr_p.divide_matrix(15, 4) == ([0, 3, 6, 9], [3, 3, 3, 6])
+    assert bcr_p.get_process_rowblock_index(6, 3) == (6, 9)
