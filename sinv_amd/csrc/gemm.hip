@@ -184,13 +184,34 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
   struct Regs {
     cplx ra[EA], rb[EB];
     bool oka[EA], okb[EB];
+    bool fullA, fullB;  // whole chunk in range: no masks (wave-uniform)
   };
   Regs R0;
+  // per-element pointers at k = 0, advanced by k0 (or k0 * ld) per chunk
+  const cplx* pa[EA];
+  const cplx* pb[EB];
+  const bool fullM = tm.m0 + BMt <= g.M, fullN = tm.n0 + BNt <= g.N;
+#pragma unroll
+  for (int j = 0; j < EA; ++j)
+    pa[j] = a_kcontig ? A + (long long)(tm.m0 + am[j]) * g.lda + ak[j]
+                      : A + (long long)ak[j] * g.lda + (tm.m0 + am[j]);
+#pragma unroll
+  for (int j = 0; j < EB; ++j)
+    pb[j] = b_ncontig ? B + (long long)bk[j] * g.ldb + (tm.n0 + bn[j])
+                      : B + (long long)(tm.n0 + bn[j]) * g.ldb + bk[j];
+  const long long a_kstep = a_kcontig ? 1 : g.lda;  // elements per unit of k
+  const long long b_kstep = b_ncontig ? g.ldb : 1;
 
   // Branch-free tile loads: an out-of-range element is read from (0, 0), which
   // always exists, and zeroed when staged into LDS.  All loads of a chunk are
   // then in flight together instead of each behind its own branch.
   auto load_tiles = [&](Regs& X, int k0) {
+    X.fullA = fullM && k0 + BK <= g.K;
+    X.fullB = fullN && k0 + BK <= g.K;
+    if (X.fullA) {
+#pragma unroll
+      for (int j = 0; j < EA; ++j) X.ra[j] = ld_c(pa[j] + k0 * a_kstep);
+    } else {
 #pragma unroll
     for (int j = 0; j < EA; ++j) {
       const int m = tm.m0 + am[j], k = k0 + ak[j];
@@ -198,6 +219,11 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
       const long long mm = X.oka[j] ? m : 0, kk = X.oka[j] ? k : 0;
       X.ra[j] = a_kcontig ? ld_c(A + mm * g.lda + kk) : ld_c(A + kk * g.lda + mm);
     }
+    }
+    if (X.fullB) {
+#pragma unroll
+      for (int j = 0; j < EB; ++j) X.rb[j] = ld_c(pb[j] + k0 * b_kstep);
+    } else {
 #pragma unroll
     for (int j = 0; j < EB; ++j) {
       const int k = k0 + bk[j], n = tm.n0 + bn[j];
@@ -205,21 +231,34 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
       const long long kk = X.okb[j] ? k : 0, nn = X.okb[j] ? n : 0;
       X.rb[j] = b_ncontig ? ld_c(B + kk * g.ldb + nn) : ld_c(B + nn * g.ldb + kk);
     }
+    }
   };
   auto store_tiles = [&](const Regs& X, int buf) {
     auto& As = As2[buf];
     auto& Bs = Bs2[buf];
+    if (X.fullA) {
+#pragma unroll
+      for (int j = 0; j < EA; ++j)
+        As[ak[j]][am[j]] = CR<R>{(R)X.ra[j].re, (R)(X.ra[j].im * a_conj)};
+    } else {
 #pragma unroll
     for (int j = 0; j < EA; ++j) {
       const double re = X.oka[j] ? X.ra[j].re : 0.0;
       const double im = X.oka[j] ? X.ra[j].im * a_conj : 0.0;
       As[ak[j]][am[j]] = CR<R>{(R)re, (R)im};
     }
+    }
+    if (X.fullB) {
+#pragma unroll
+      for (int j = 0; j < EB; ++j)
+        Bs[bk[j]][bn[j]] = CR<R>{(R)X.rb[j].re, (R)(X.rb[j].im * b_conj)};
+    } else {
 #pragma unroll
     for (int j = 0; j < EB; ++j) {
       const double re = X.okb[j] ? X.rb[j].re : 0.0;
       const double im = X.okb[j] ? X.rb[j].im * b_conj : 0.0;
       Bs[bk[j]][bn[j]] = CR<R>{(R)re, (R)im};
+    }
     }
   };
 
