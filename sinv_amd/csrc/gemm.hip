@@ -236,10 +236,12 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
   auto store_tiles = [&](const Regs& X, int buf) {
     auto& As = As2[buf];
     auto& Bs = Bs2[buf];
-    if (X.fullA) {
+    if (X.fullA && g.opA != OP_C) {
 #pragma unroll
-      for (int j = 0; j < EA; ++j)
-        As[ak[j]][am[j]] = CR<R>{(R)X.ra[j].re, (R)(X.ra[j].im * a_conj)};
+      for (int j = 0; j < EA; ++j) As[ak[j]][am[j]] = CR<R>{(R)X.ra[j].re, (R)X.ra[j].im};
+    } else if (X.fullA) {
+#pragma unroll
+      for (int j = 0; j < EA; ++j) As[ak[j]][am[j]] = CR<R>{(R)X.ra[j].re, (R)(-X.ra[j].im)};
     } else {
 #pragma unroll
     for (int j = 0; j < EA; ++j) {
@@ -248,10 +250,12 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
       As[ak[j]][am[j]] = CR<R>{(R)re, (R)im};
     }
     }
-    if (X.fullB) {
+    if (X.fullB && g.opB != OP_C) {
 #pragma unroll
-      for (int j = 0; j < EB; ++j)
-        Bs[bk[j]][bn[j]] = CR<R>{(R)X.rb[j].re, (R)(X.rb[j].im * b_conj)};
+      for (int j = 0; j < EB; ++j) Bs[bk[j]][bn[j]] = CR<R>{(R)X.rb[j].re, (R)X.rb[j].im};
+    } else if (X.fullB) {
+#pragma unroll
+      for (int j = 0; j < EB; ++j) Bs[bk[j]][bn[j]] = CR<R>{(R)X.rb[j].re, (R)(-X.rb[j].im)};
     } else {
 #pragma unroll
     for (int j = 0; j < EB; ++j) {
@@ -348,6 +352,57 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
   cplx sh{0.0, 0.0};
   const bool use_shift = g.shift != nullptr;
   if (use_shift) sh = g.shift[tm.z_out];
+  // complex result of accumulator item (i, j, v)
+  auto result = [&](int i, int j, int v, double& xr, double& xi) {
+    if constexpr (kThreeM<R>) {
+      const double p1 = (double)t1[i][j][v], p2 = (double)t2[i][j][v];
+      xr = p1 - p2;
+      xi = (double)t3[i][j][v] - p1 - p2;
+    } else {
+      xr = (double)t1[i][j][v];
+      xi = (double)t3[i][j][v];
+    }
+  };
+  if (fullM && fullN) {
+    // interior tile: no bounds checks, all C0 loads issued before any use
+    cplx c0v[TM][TN][4];
+    if (use_c0) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = tm.m0 + wm * TM * 16 + i * 16 + acc_row<R>(lane, v);
+            const int col = tm.n0 + wn * TN * 16 + j * 16 + (lane & 15);
+            c0v[i][j][v] = ld_c(C0 + (long long)row * g.ldc0 + col);
+          }
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = tm.m0 + wm * TM * 16 + i * 16 + acc_row<R>(lane, v);
+          const int col = tm.n0 + wn * TN * 16 + j * 16 + (lane & 15);
+          double xr, xi;
+          result(i, j, v, xr, xi);
+          double outr = g.alpha.re * xr - g.alpha.im * xi;
+          double outi = g.alpha.re * xi + g.alpha.im * xr;
+          if (use_c0) {
+            const cplx c0 = c0v[i][j][v];
+            outr += g.beta.re * c0.re - g.beta.im * c0.im;
+            outi += g.beta.re * c0.im + g.beta.im * c0.re;
+          }
+          if (use_shift && row == col) {
+            outr += sh.re;
+            outi += sh.im;
+          }
+          *reinterpret_cast<double2*>(Cp + (long long)row * g.ldc + col) = double2{outr, outi};
+        }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -358,14 +413,7 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
         const int col = tm.n0 + wn * TN * 16 + j * 16 + (lane & 15);
         if (row < g.M && col < g.N) {
           double xr, xi;
-          if constexpr (kThreeM<R>) {
-            const double p1 = (double)t1[i][j][v], p2 = (double)t2[i][j][v];
-            xr = p1 - p2;
-            xi = (double)t3[i][j][v] - p1 - p2;
-          } else {
-            xr = (double)t1[i][j][v];
-            xi = (double)t3[i][j][v];
-          }
+          result(i, j, v, xr, xi);
           double outr = g.alpha.re * xr - g.alpha.im * xi;
           double outi = g.alpha.re * xi + g.alpha.im * xr;
           if (use_c0) {
