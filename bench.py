@@ -346,6 +346,11 @@ def main():
                 kernel=kname, launches=kcnt, avg_launch_us=round(kms / max(kcnt, 1) * 1e3, 3),
                 flops_per_launch=kfl / max(kcnt, 1),
                 algorithmic_bytes_per_launch=results[head]["bytes"][kind] / max(kcnt, 1),
+                # FP64 complex products run as 3 real MFMA products (3M) where 8MNK counts 4:
+                # the matrix cores execute 3/4 of the algorithmic flops
+                mfma_executed_tflops=round(achieved * (0.75 if head == "fp64" else 1.0), 3),
+                mfma_executed_frac=round(achieved * (0.75 if head == "fp64" else 1.0)
+                                         / PEAK_TFLOPS[kind], 4),
                 traffic_source=f"profiles/traffic_{wl['workload']}.txt (rocprofv3 --pmc "
                                "FETCH_SIZE + WRITE_SIZE, calibrated; null if absent)")
     out = {
