@@ -438,11 +438,13 @@ using Cfg32w4 = Cfg<2, 2, 1, 1, 4>;   // 1: 32 x 32, 4 waves of 16 x 16
 using Cfg32w1 = Cfg<1, 1, 2, 2, 4>;   // 2: 32 x 32, 1 wave
 using Cfg128x64 = Cfg<4, 2, 2, 2, 2>; // 3: 128 x 64, 8 waves
 using Cfg64x128 = Cfg<2, 4, 2, 2, 2>; // 4: 64 x 128, 8 waves
-constexpr int kNumCfg = 5;
-constexpr int kCfgBM[kNumCfg] = {Cfg64::BM, Cfg32w4::BM, Cfg32w1::BM, Cfg128x64::BM,
-                                 Cfg64x128::BM};
-constexpr int kCfgBN[kNumCfg] = {Cfg64::BN, Cfg32w4::BN, Cfg32w1::BN, Cfg128x64::BN,
-                                 Cfg64x128::BN};
+using Cfg64x32 = Cfg<2, 2, 2, 1, 3>;  // 5: 64 x 32, 4 waves of 32 x 16 (3 workgroups per CU)
+using Cfg32x64 = Cfg<2, 2, 1, 2, 3>;  // 6: 32 x 64, 4 waves of 16 x 32
+constexpr int kNumCfg = 7;
+constexpr int kCfgBM[kNumCfg] = {Cfg64::BM,     Cfg32w4::BM,   Cfg32w1::BM, Cfg128x64::BM,
+                                 Cfg64x128::BM, Cfg64x32::BM, Cfg32x64::BM};
+constexpr int kCfgBN[kNumCfg] = {Cfg64::BN,     Cfg32w4::BN,   Cfg32w1::BN, Cfg128x64::BN,
+                                 Cfg64x128::BN, Cfg64x32::BN, Cfg32x64::BN};
 
 template <typename R, typename C>
 void launch_cfg(const GemmArgs& g, long long batch, hipStream_t s) {
@@ -459,6 +461,8 @@ void launch_by_id(int id, const GemmArgs& g, long long batch, hipStream_t s) {
     case 2: launch_cfg<R, Cfg32w1>(g, batch, s); break;
     case 3: launch_cfg<R, Cfg128x64>(g, batch, s); break;
     case 4: launch_cfg<R, Cfg64x128>(g, batch, s); break;
+    case 5: launch_cfg<R, Cfg64x32>(g, batch, s); break;
+    case 6: launch_cfg<R, Cfg32x64>(g, batch, s); break;
     default: launch_cfg<R, Cfg64>(g, batch, s); break;
   }
 }
