@@ -1,11 +1,13 @@
 // K1: batched in-place complex128 block inverse.
 //
 // Leaves (n <= 64): one workgroup per matrix, Gauss-Jordan elimination with
-// partial pivoting.  The pivot is the first row maximising |re| + |im| in the
-// column (the choice LAPACK's izamax makes inside ?getrf, which
-// numpy.linalg.inv calls).  Default kernel: the block lives in registers, one
-// row per lane (inv_leaf_kernel_rows); the older LDS-resident kernel
-// (inv_leaf_kernel) is kept as a selectable alternative.
+// partial pivoting on |re| + |im| (the modulus LAPACK's izamax uses inside
+// ?getrf, which numpy.linalg.inv calls).  Default kernel: the block lives in
+// registers, one row per lane, and the steps run panel by panel, one wave
+// per panel (inv_leaf_kernel_panel; the pivot is the earliest row within
+// 2^-15 of the column maximum).  The per-step register kernel
+// (inv_leaf_kernel_rows, exact first maximum) and the older LDS-resident
+// kernel (inv_leaf_kernel) are kept as selectable alternatives.
 //
 // Larger blocks: 2 x 2 Schur-complement recursion, all of whose arithmetic is
 // batched GEMM on FP64 MFMA:
@@ -326,19 +328,245 @@ __global__ __launch_bounds__(64 * W) void inv_leaf_kernel_rows(cplx* A, int n, i
   }
 }
 
-// Leaf kernel choice: the row-layout register kernel with 8 waves unless
-// SINV_LEAF_KERNEL names another ("lds": the LDS-resident kernel, "rows4":
-// the register kernel with 4 waves).  Read once per process.
+// Complex reciprocal of a wave-uniform value, without a division: the power
+// of two sc = 2^(1023 - e), e the larger biased exponent of re and im (SALU
+// bit arithmetic), brings the larger part into [1, 2) exactly; the squared
+// modulus d in [1, 8) is inverted with v_rcp_f64 and two Newton steps; the
+// result is sc (re' - i im') / d.  Error a few ulp, like Smith's method, at
+// about a third of its latency.
+__device__ __forceinline__ cplx crecip_uniform(cplx p) {
+  const unsigned hre = (unsigned)((unsigned long long)__double_as_longlong(p.re) >> 32) & 0x7FFFFFFFu;
+  const unsigned him = (unsigned)((unsigned long long)__double_as_longlong(p.im) >> 32) & 0x7FFFFFFFu;
+  const unsigned e = min((hre > him ? hre : him) >> 20, 2045u);
+  const double sc = __longlong_as_double((long long)(2046u - e) << 52);
+  const double re = p.re * sc, im = p.im * sc;
+  const double d = fma(re, re, im * im);
+  double q = __builtin_amdgcn_rcp(d);
+  q = fma(q, fma(-d, q, 1.0), q);
+  q = fma(q, fma(-d, q, 1.0), q);
+  q *= sc;
+  return cplx{re * q, -im * q};
+}
+
+// Wave64 maximum of a 32-bit key, left in lane 63: six v_max_u32 with DPP
+// sources (quad swaps, half-row and row mirrors, row broadcasts 15 and 31).
+// Each DPP read follows the write of its source by the required two wait
+// states; lanes outside a row mask keep their value.
+__device__ __forceinline__ unsigned wave_umax(unsigned x) {
+  asm volatile(
+      "s_nop 1\n\t"
+      "v_max_u32_dpp %0, %0, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_u32_dpp %0, %0, %0 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_u32_dpp %0, %0, %0 row_half_mirror row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_u32_dpp %0, %0, %0 row_mirror row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_u32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_u32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+      "s_nop 1"
+      : "+v"(x));
+  return x;
+}
+
+// Pivot of one elimination step, as one wave-wide max of a 32-bit key: the
+// high word of the modulus |re| + |im| (exponent and 20 mantissa bits;
+// raised to at least the smallest normal double) with its 5 lowest bits
+// replaced by 63 - position.  The winner is the earliest row, in position
+// order, whose modulus is within 2^-15 relative of the column maximum
+// (LAPACK's izamax takes the exact first maximum; the two choices coincide
+// unless two candidates are that close, and either bounds the multipliers by
+// 1 + 2^-15).  Rows already used, padding lanes and NaN moduli do not take
+// part; with no candidate at all the row at position k is kept.  Returns the
+// pivot lane; keys are distinct, so the lane is found by comparing keys.
+__device__ __forceinline__ int pivot_lane(cplx f, int pos, int k, int lane, int n) {
+  const double m = fabs(f.re) + fabs(f.im);
+  const unsigned long long mbits = (unsigned long long)__double_as_longlong(m);
+  const unsigned hi = (unsigned)(mbits >> 32);
+  const bool cand = mbits <= 0x7FF0000000000000ull && pos >= k && lane < n;  // NaN excluded
+  const unsigned hk = (max(hi, 0x00100000u) << 1) & ~63u;
+  const unsigned mykey = cand ? (hk | (unsigned)(63 - pos)) : 0u;
+  const unsigned top = (unsigned)__builtin_amdgcn_readlane((int)wave_umax(mykey), 63);
+  const unsigned long long hit = __ballot(mykey == top);
+  const unsigned long long keep_k = __ballot(pos == k);
+  return (int)__builtin_ctzll(top != 0 ? hit : keep_k);
+}
+
+// Panel form of the register leaf: the same row layout and update formula as
+// inv_leaf_kernel_rows, with the steps grouped by the wave that owns their
+// columns (W waves of R = 64/W columns).  The owner of panel kb runs its R
+// steps alone -- pivot search (pivot_lane), 1/pivot (crecip_uniform) and
+// the update of its own columns, the next pivot column first -- as straight-
+// line code, and publishes each step's multiplier vector g, pivot row and
+// position; after ONE barrier per panel every other wave applies the R
+// published rank-1 updates to its columns, reading the pivot row's entries
+// from its own registers (lane p) as each step reaches them.  Every element
+// receives the same updates in the same order as in a step-by-step
+// elimination; the barriers drop from n to W.  Wave priorities follow the
+// critical path: the factoring wave first, then the next panel's owner while
+// it applies, then everyone else.
+template <int R, bool FULL>
+__device__ __forceinline__ void leaf_factor_panel(cplx (&a)[R], int kb, int n, int lane, int& pos,
+                                                  bool& singular, cplx (*gb)[64], int* pb,
+                                                  int* pivpos) {
+  int p_of = 0, pp_of = 0;  // lane kr keeps step kr's pivot row and position
+#pragma unroll
+  for (int kr = 0; kr < R; ++kr) {
+    const int k = kb * R + kr;
+    if (FULL || k < n) {
+      const cplx f = a[kr];
+      const int p = pivot_lane(f, pos, k, lane, n);
+      const int pp = __builtin_amdgcn_readlane(pos, p);
+      const cplx piv{readlane_f64(f.re, p), readlane_f64(f.im, p)};
+      singular |= piv.re == 0.0 && piv.im == 0.0;
+      const cplx d = crecip_uniform(piv);
+      const bool is_p = lane == p;
+      const cplx gf = cmul(f, d);
+      const cplx g{is_p ? -d.re : gf.re, is_p ? -d.im : gf.im};
+      const double keep = is_p ? 0.0 : 1.0;
+      gb[kr][lane] = g;
+      p_of = lane == kr ? p : p_of;
+      pp_of = lane == kr ? pp : pp_of;
+#pragma unroll
+      for (int j = 1; j <= R; ++j) {
+        const int r = (kr + j) % R;  // column kr + 1 (the next pivot column) first
+        cplx P{readlane_f64(a[r].re, p), readlane_f64(a[r].im, p)};
+        cplx av = a[r];
+        if (r == kr) {  // column k: cleared, unit entry in P
+          P = cplx{1.0, 0.0};
+          av = cplx{0.0, 0.0};
+        }
+        a[r] = cplx{keep * av.re - (g.re * P.re - g.im * P.im),
+                    keep * av.im - (g.re * P.im + g.im * P.re)};
+      }
+      pos = pos == k ? pp : (is_p ? k : pos);
+    }
+  }
+  if (lane < R && (FULL || kb * R + lane < n)) {
+    pb[lane] = p_of;
+    pivpos[kb * R + lane] = pp_of;
+  }
+}
+
+template <int R, bool FULL>
+__device__ __forceinline__ void leaf_apply_panel(cplx (&a)[R], int kb, int n, int lane, int& pos,
+                                                 const cplx (*gb)[64], const int* pb,
+                                                 const int* pivpos) {
+#pragma unroll
+  for (int kr = 0; kr < R; ++kr) {
+    const int k = kb * R + kr;
+    if (FULL || k < n) {
+      const int p = __builtin_amdgcn_readfirstlane(pb[kr]);
+      const int pp = __builtin_amdgcn_readfirstlane(pivpos[k]);
+      const cplx g = gb[kr][lane];
+      const double keep = lane == p ? 0.0 : 1.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const cplx P{readlane_f64(a[r].re, p), readlane_f64(a[r].im, p)};
+        a[r] = cplx{keep * a[r].re - (g.re * P.re - g.im * P.im),
+                    keep * a[r].im - (g.re * P.im + g.im * P.re)};
+      }
+      pos = pos == k ? pp : (lane == p ? k : pos);
+    }
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(64 * W) void inv_leaf_kernel_panel(cplx* A, int n, int lda,
+                                                                long long s_in, long long s_out,
+                                                                int inner,
+                                                                unsigned long long* info,
+                                                                int tag) {
+  constexpr int R = 64 / W;
+  __shared__ cplx gbuf[2][R][64];  // per panel parity: multiplier vector of each step
+  __shared__ int pbuf[2][R];       // pivot row of each step
+  __shared__ int pivpos[64];
+  __shared__ int invp[64];
+
+  const int z = blockIdx.x;
+  const int zo = z / inner, zi = z - zo * inner;
+  A += zi * s_in + zo * s_out;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c0 = w * R;
+
+  cplx a[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int c = c0 + r;
+    if (lane < n && c < n) {
+      const double2 v = *reinterpret_cast<const double2*>(A + (long long)lane * lda + c);
+      a[r] = cplx{v.x, v.y};
+    } else {
+      a[r] = cplx{lane == c ? 1.0 : 0.0, 0.0};
+    }
+  }
+
+  int pos = lane;  // position of this row in the swapped row order
+  bool singular = false;
+  const int npanels = (n + R - 1) / R;
+  for (int kb = 0; kb < npanels; ++kb) {
+    const int buf = kb & 1;
+    const bool full = kb * R + R <= n;
+    if (w == kb) {
+      __builtin_amdgcn_s_setprio(3);
+      if (full)
+        leaf_factor_panel<R, true>(a, kb, n, lane, pos, singular, gbuf[buf], pbuf[buf], pivpos);
+      else
+        leaf_factor_panel<R, false>(a, kb, n, lane, pos, singular, gbuf[buf], pbuf[buf], pivpos);
+    }
+    __syncthreads();
+    if (w != kb) {
+      // the next panel's owner applies first: it is on the critical path
+      if (w == kb + 1)
+        __builtin_amdgcn_s_setprio(2);
+      else
+        __builtin_amdgcn_s_setprio(0);
+      if (full)
+        leaf_apply_panel<R, true>(a, kb, n, lane, pos, gbuf[buf], pbuf[buf], pivpos);
+      else
+        leaf_apply_panel<R, false>(a, kb, n, lane, pos, gbuf[buf], pbuf[buf], pivpos);
+    }
+  }
+
+  // inv(A) = inv(P A) P: column c of the eliminated matrix is column invp[c]
+  // of the result, where invp inverts the final row positions
+  if (w == 0 && lane < n) invp[pos] = lane;
+  if (lane == 0 && singular && info != nullptr)
+    atomicMin(info, ((unsigned long long)(unsigned)tag << 32) | (unsigned long long)(unsigned)z);
+  __syncthreads();
+  if (lane < n) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int c = c0 + r;
+      if (c < n)
+        *reinterpret_cast<double2*>(A + (long long)pos * lda + invp[c]) =
+            double2{a[r].re, a[r].im};
+    }
+  }
+}
+
+// Leaf kernel choice: the panel kernel, with 16 waves of 4 columns when the
+// launch leaves CUs idle (fewer than two matrices per CU: the per-step
+// latency of the factoring wave is what counts) and 8 waves of 8 columns
+// otherwise (fewer, cheaper barriers; workgroups share CUs).  SINV_LEAF_KERNEL
+// forces one ("panel16", "panel8", "rows8": the register kernel with one
+// barrier per step, "lds": the LDS-resident kernel).  Read once per process.
 int leaf_variant() {
   static int v = -1;
   if (v < 0) {
     const char* e = getenv("SINV_LEAF_KERNEL");
-    v = 2;
+    v = 1;
     if (e != nullptr && strcmp(e, "lds") == 0) v = 0;
-    if (e != nullptr && strcmp(e, "rows4") == 0) v = 1;
+    if (e != nullptr && strcmp(e, "rows8") == 0) v = 2;
+    if (e != nullptr && strcmp(e, "panel8") == 0) v = 3;
+    if (e != nullptr && strcmp(e, "panel16") == 0) v = 4;
   }
   return v;
 }
+constexpr int kLeafPanel8MinGrid = 512;  // 2 x the 256 CUs of MI355X
 
 // Split so that the recursion's GEMM dimensions stay multiples of the 64-wide
 // GEMM tile where possible: the largest multiple of 64 not above n/2, else a
@@ -367,7 +595,9 @@ hipError_t launch_inverse(cplx* A, int n, int lda, long long s_in, long long s_o
   if (n <= LEAF) {
     const int pid = prof_open(s);
     const dim3 grid((unsigned)(inner * outer));
-    switch (leaf_variant()) {
+    int v = leaf_variant();
+    if (v == 1) v = grid.x >= (unsigned)kLeafPanel8MinGrid ? 3 : 4;
+    switch (v) {
       case 0:
         hipLaunchKernelGGL(inv_leaf_kernel, grid, dim3(256), 0, s, A, n, lda, s_in, s_out, inner,
                            info, tag);
@@ -376,8 +606,12 @@ hipError_t launch_inverse(cplx* A, int n, int lda, long long s_in, long long s_o
         hipLaunchKernelGGL(inv_leaf_kernel_rows<8>, grid, dim3(512), 0, s, A, n, lda, s_in,
                            s_out, inner, info, tag);
         break;
+      case 3:
+        hipLaunchKernelGGL(inv_leaf_kernel_panel<8>, grid, dim3(512), 0, s, A, n, lda, s_in,
+                           s_out, inner, info, tag);
+        break;
       default:
-        hipLaunchKernelGGL(inv_leaf_kernel_rows<4>, grid, dim3(256), 0, s, A, n, lda, s_in,
+        hipLaunchKernelGGL(inv_leaf_kernel_panel<16>, grid, dim3(1024), 0, s, A, n, lda, s_in,
                            s_out, inner, info, tag);
     }
     prof_close(pid, s, PROF_INV_LEAF, 8.0 * n * (double)n * n * inner * (double)outer,
