@@ -58,7 +58,8 @@ struct PsrPlan {
   long long Ad = 0, invA = 0, Lf = 0, Uf = 0, Ls = 0, Us = 0, Afc = 0, Afr = 0, Gsr = 0, Gsc = 0;
   long long Rd = 0, Ru = 0, Rl = 0;  // reduced system (R, R-1, R-1)
   long long per_inst = 0;
-  size_t inv_elems = 0;
+  size_t inv_elems = 0;       // inverse workspace of the central partitions
+  size_t inv_side_elems = 0;  // each of the top and bottom partitions
   size_t rgf_bytes = 0;
   size_t bytes = 0;
 };
@@ -84,6 +85,7 @@ PsrPlan psr_plan(int N, int b, int P, int batch, int first_rows) {
   Q.per_inst = o;
   const int maxc = P - 2 > 1 ? P - 2 : 1;
   Q.inv_elems = inverse_workspace_elems(b, batch * maxc);
+  Q.inv_side_elems = inverse_workspace_elems(b, batch);
   sinv_rgf_desc rd{};
   rd.nblocks = Q.R; rd.blocksize = b; rd.batch = batch; rd.precision = SINV_PREC_FP64;
   rd.save_off_diag = 1;
@@ -94,7 +96,8 @@ PsrPlan psr_plan(int N, int b, int P, int batch, int first_rows) {
   Q.rgf_bytes = sinv_rgf_workspace_bytes(&rd);
   const size_t b2 = (size_t)b * b;
   const size_t red_out = (size_t)(3 * Q.R - 2) * b2 * batch;
-  Q.bytes = ((size_t)Q.per_inst * b2 * batch + red_out + Q.inv_elems + 1) * sizeof(cplx) +
+  Q.bytes = ((size_t)Q.per_inst * b2 * batch + red_out + Q.inv_elems + 2 * Q.inv_side_elems + 1) *
+                sizeof(cplx) +
             Q.rgf_bytes + 1024;
   return Q;
 }
@@ -128,7 +131,9 @@ extern "C" int sinv_psr_batched(const sinv_psr_desc* d, void* work, size_t work_
   cplx* GRu = GRd + (long long)Q.R * b2 * nb;
   cplx* GRl = GRu + (long long)(Q.R - 1) * b2 * nb;
   cplx* winv = GRl + (long long)(Q.R - 1) * b2 * nb;
-  unsigned long long* rinfo = (unsigned long long*)(winv + Q.inv_elems);
+  cplx* winv_top = winv + Q.inv_elems;
+  cplx* winv_bot = winv_top + Q.inv_side_elems;
+  unsigned long long* rinfo = (unsigned long long*)(winv_bot + Q.inv_side_elems);
   void* wrgf = (void*)(((uintptr_t)(rinfo + 1) + 255) & ~(uintptr_t)255);
 
   const cplx one{1.0, 0.0}, mone{-1.0, 0.0}, zero{0.0, 0.0};
@@ -150,6 +155,11 @@ extern "C" int sinv_psr_batched(const sinv_psr_desc* d, void* work, size_t work_
   const Arr Up{(const cplx*)d->upper, d->upper_stride}, Lo{(const cplx*)d->lower, d->lower_stride};
   const Arr GD{Gd, GdI}, GU{Gu, GoI}, GL{Gl, GoI};
 
+  // The top, bottom and central partitions are independent between the
+  // exchanges: each runs on its own stream (cur) with its own inverse
+  // workspace (wcur), as the reference's ranks run concurrently.
+  hipStream_t cur = s;
+  cplx* wcur = winv;
   // C[k] (k < count; block index c0 + k*cs) = alpha A[a0 + k*as] B[b0 + k*bs] (+ C0[...])
   auto gemm = [&](int count, Arr A, long long a0, long long as, Arr B, long long b0,
                   long long bs, Arr C, long long c0, long long cs, cplx alpha, bool acc,
@@ -171,20 +181,20 @@ extern "C" int sinv_psr_batched(const sinv_psr_desc* d, void* work, size_t work_
     g.alpha = alpha;
     g.shift = nullptr;
     g.inner = count; g.outer = nb;
-    return launch_gemm(g, Prec::F64, s);
+    return launch_gemm(g, Prec::F64, cur);
   };
   auto copy = [&](int count, Arr S, long long s0, long long ss, Arr C, long long c0, long long cs,
                   cplx a) {
     if (count <= 0) return hipSuccess;
     return launch_axpby_shift(b, b, a, S.p + s0 * b2, b, ss * b2, S.inst,
                               const_cast<cplx*>(C.p) + c0 * b2, b, cs * b2, C.inst, nullptr,
-                              count, nb, s);
+                              count, nb, cur);
   };
   // invA[r + k*st] = inv(Ad[r + k*st])
   auto invert = [&](int count, long long r, long long st) -> hipError_t {
     SINV_HIP_RET(copy(count, Ad, r, st, invA, r, st, one));
-    return launch_inverse(W + (Q.invA + r) * b2, b, b, st * b2, WI, count, nb, winv, d->info,
-                          (int)r, s);
+    return launch_inverse(W + (Q.invA + r) * b2, b, b, st * b2, WI, count, nb, wcur, d->info,
+                          (int)r, cur);
   };
 
   if (d->info != nullptr) SINV_HIP_TRY(launch_fill_u64(d->info, 1, SINV_INFO_OK, s));
@@ -200,7 +210,11 @@ extern "C" int sinv_psr_batched(const sinv_psr_desc* d, void* work, size_t work_
   const int eB = N - 1;
 
   // ---------------------------------------------------------------- reduction
+  StreamFork red_fork(s, 2);
+  SINV_HIP_TRY(red_fork.status());
   // top partition, downwards (psr_seqsolve.py:157-167)
+  cur = red_fork.side(0);
+  wcur = winv_top;
   for (int i = 1; i < n0; ++i) {
     SINV_HIP_TRY(invert(1, i - 1, 0));
     SINV_HIP_TRY(gemm(1, Lo, i - 1, 0, invA, i - 1, 0, Lf, i, 0, osc, false));
@@ -208,6 +222,8 @@ extern "C" int sinv_psr_batched(const sinv_psr_desc* d, void* work, size_t work_
     SINV_HIP_TRY(gemm(1, Lf, i, 0, Up, i - 1, 0, Ad, i, 0, mosc, true));
   }
   // bottom partition, upwards (psr_seqsolve.py:203-216)
+  cur = red_fork.side(1);
+  wcur = winv_bot;
   for (int i = eB - 1; i >= sB; --i) {
     SINV_HIP_TRY(invert(1, i + 1, 0));
     SINV_HIP_TRY(gemm(1, Up, i, 0, invA, i + 1, 0, Lf, i, 0, osc, false));
@@ -215,6 +231,8 @@ extern "C" int sinv_psr_batched(const sinv_psr_desc* d, void* work, size_t work_
     SINV_HIP_TRY(gemm(1, Lf, i, 0, Lo, i, 0, Ad, i, 0, mosc, true));
   }
   // central partitions, all at once (psr_seqsolve.py:260-307); local row t, r = t - 1
+  cur = s;
+  wcur = winv;
   for (int t = 2; t < m; ++t) {
     const long long i = s1 + t, r = s1 + t - 1, sr = s1;
     SINV_HIP_TRY(invert(C, r, m));
@@ -242,6 +260,8 @@ extern "C" int sinv_psr_batched(const sinv_psr_desc* d, void* work, size_t work_
     // A_{s,i} = -L_{s,r} A_{r,i}
     SINV_HIP_TRY(gemm(C, Ls, r, m, Up, r, m, Afr, i, m, mosc, false));
   }
+
+  SINV_HIP_TRY(red_fork.join());
 
   // ---------------------------------------------------------------- reduced system
   const Arr Rd = warr(Q.Rd), Ru = warr(Q.Ru), Rl = warr(Q.Rl);
@@ -292,19 +312,24 @@ extern "C" int sinv_psr_batched(const sinv_psr_desc* d, void* work, size_t work_
   }
 
   // ---------------------------------------------------------------- production
+  StreamFork prod_fork(s, 2);
+  SINV_HIP_TRY(prod_fork.status());
   // top partition, upwards (psr_seqsolve.py:731-740)
+  cur = prod_fork.side(0);
   for (int i = n0 - 1; i >= 1; --i) {
     SINV_HIP_TRY(gemm(1, GD, i, 0, Lf, i, 0, GL, i - 1, 0, mone, false));
     SINV_HIP_TRY(gemm(1, Uf, i, 0, GD, i, 0, GU, i - 1, 0, mone, false));
     SINV_HIP_TRY(gemm(1, Uf, i, 0, GL, i - 1, 0, GD, i - 1, 0, mone, true, invA, i - 1, 0));
   }
   // bottom partition, downwards (psr_seqsolve.py:777-786)
+  cur = prod_fork.side(1);
   for (int i = sB; i < eB; ++i) {
     SINV_HIP_TRY(gemm(1, GD, i, 0, Lf, i, 0, GU, i, 0, mone, false));
     SINV_HIP_TRY(gemm(1, Uf, i, 0, GD, i, 0, GL, i, 0, mone, false));
     SINV_HIP_TRY(gemm(1, Uf, i, 0, GU, i, 0, GD, i + 1, 0, mone, true, invA, i + 1, 0));
   }
   // central partitions (psr_seqsolve.py:823-898); s = s1 + q m, e = s + m - 1
+  cur = s;
   if (C > 0) {
     const long long sr = s1, e = s1 + m - 1;
     // G_{e,e-1} = -G_{e,s} L_{s,e-1} - G_ee L_{e,e-1}
@@ -339,5 +364,6 @@ extern "C" int sinv_psr_batched(const sinv_psr_desc* d, void* work, size_t work_
     SINV_HIP_TRY(copy(C, Gsr, sr + 1, m, GU, sr, m, one));
     SINV_HIP_TRY(copy(C, Gsc, sr + 1, m, GL, sr, m, one));
   }
+  SINV_HIP_TRY(prod_fork.join());
   return SINV_OK;
 }

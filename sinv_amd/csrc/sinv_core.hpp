@@ -109,6 +109,33 @@ inline BlockLayout make_block_layout(int N, int blocksize, const int* bsizes) {
   return L;
 }
 
+// Fork/join of independent launch sequences onto library-owned side streams
+// (default priority, non-blocking, created once per device on first use).
+// The constructor makes nside side streams wait for the work already queued
+// on `main`; join() (or the destructor) makes `main` wait for everything
+// queued on them since.  Stream capture follows the fork and the join like
+// any cross-stream event dependency.  Callers that fork concurrently share
+// the side streams: their side work serialises, their results do not mix.
+class StreamFork {
+ public:
+  static constexpr int kMaxSide = 2;
+  StreamFork(hipStream_t main, int nside);
+  ~StreamFork() { join(); }
+  StreamFork(const StreamFork&) = delete;
+  StreamFork& operator=(const StreamFork&) = delete;
+  hipStream_t side(int i) const { return side_[i]; }
+  hipError_t status() const { return err_; }
+  hipError_t join();
+
+ private:
+  hipStream_t main_;
+  int n_ = 0;
+  hipStream_t side_[kMaxSide] = {};
+  hipEvent_t fork_ = nullptr;
+  bool joined_ = false;
+  hipError_t err_ = hipSuccess;
+};
+
 // thread-local last error text for sinv_last_error()
 void set_error(const char* fmt, ...);
 

@@ -83,6 +83,46 @@ def test_zinv_needs_pivoting(built_lib, n):
         assert err < 1e-15 * np.linalg.cond(A[z]) * n, (z, err)
 
 
+@pytest.mark.parametrize("n", [37, 64])
+def test_zinv_leaf_both_panel_widths(built_lib, n):
+    """The leaf kernel uses 16 waves of 4 columns for launches under 512
+    matrices and 8 waves of 8 columns above: both give numpy's inverse, and
+    agree with each other to rounding."""
+    from sinv_amd import core
+    rng = np.random.default_rng(7 + n)
+    A = _rand(rng, 600, n, n)
+    X_big = core.zinv_batched(A).cpu().numpy()      # 600 matrices: 8-wave panels
+    X_small = core.zinv_batched(A[:8]).cpu().numpy()  # 8 matrices: 16-wave panels
+    ref = np.linalg.inv(A[:8])
+    for z in range(8):
+        tol = 1e-15 * np.linalg.cond(A[z]) * n
+        assert np.linalg.norm(X_small[z] - ref[z]) / np.linalg.norm(ref[z]) < tol, z
+        assert np.linalg.norm(X_big[z] - ref[z]) / np.linalg.norm(ref[z]) < tol, z
+    zs = rng.choice(600, 16, replace=False)
+    err = max(np.linalg.norm(X_big[z] @ A[z] - np.eye(n)) for z in zs)
+    assert err < 1e-11, err
+
+
+def test_zinv_near_tie_pivot(built_lib):
+    """Two pivot candidates whose moduli differ by 1e-7 relative (inside the
+    kernel's 2^-15 comparison window, where it takes the earlier row and
+    LAPACK the larger one): the inverse is as accurate as numpy's."""
+    from sinv_amd import core
+    rng = np.random.default_rng(11)
+    n = 48
+    A = _rand(rng, 2, n, n)
+    A[:, 5, 0] = 3.0
+    A[:, 9, 0] = 3.0 * (1 + 1e-7)
+    A[1, :, 1] *= 1e-3
+    A[1, 20, 1] = 2.0 - 1e-9j
+    A[1, 30, 1] = -(2.0 + 1e-9)
+    X = core.zinv_batched(A).cpu().numpy()
+    ref = np.linalg.inv(A)
+    for z in range(2):
+        err = np.linalg.norm(X[z] - ref[z]) / np.linalg.norm(ref[z])
+        assert err < 1e-15 * np.linalg.cond(A[z]) * n, (z, err)
+
+
 def test_zinv_singular_raises(built_lib):
     from sinv_amd import core
     A = np.zeros((2, 8, 8), complex)
