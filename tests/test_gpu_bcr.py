@@ -90,15 +90,16 @@ BCRP_SHAPES = [(9, 1), (18, 2), (27, 3), (12, 1), (24, 2), (36, 3), (256, 8), (2
 @pytest.mark.parametrize("n,b", BCRP_SHAPES)
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("sym", [False, True])
-def test_bcr_parallel_dropin_reference_shapes(built_lib, n, b, cplx, sym):
-    """bcr_parallel_test.py:57-79: the tridiagonal blocks of the result against
-    the dense inverse (the reference's allclose), and bit-identical to
-    bcr_serial, whose arithmetic it shares."""
+@pytest.mark.parametrize("nparts", [1, 2])
+def test_bcr_parallel_dropin_reference_shapes(built_lib, n, b, cplx, sym, nparts):
+    """bcr_parallel_test.py:57-79 (run there on 2 ranks): the tridiagonal
+    blocks of the result against the dense inverse (the reference's
+    allclose), and bit-identical to bcr_serial, whose arithmetic it shares."""
     from sinv_amd.algorithms import bcr_p, bcr_s
     from sinv_amd.utils import matu
-    print(f"bcr_parallel n={n} b={b} cplx={cplx} sym={sym}", flush=True)
+    print(f"bcr_parallel n={n} b={b} cplx={cplx} sym={sym} nparts={nparts}", flush=True)
     A = matu.generateBandedDiagonalMatrix(n, np.ceil(b / 2), cplx, sym, 63)
-    G = bcr_p.bcr_parallel(A, b)
+    G = bcr_p.bcr_parallel(A, b, nparts)
     R = np.linalg.inv(A)
     for x, r in zip(matu.convertDenseToBlkTridiag(G, b), matu.convertDenseToBlkTridiag(R, b)):
         assert np.allclose(x, r)

@@ -86,3 +86,51 @@ def test_bcr_parallel_partitioning_helpers():
     assert bcr_p.divide_matrix(16, 4) == ([0, 4, 8, 12], [4, 4, 4, 4])
     assert bcr_p.divide_matrix(15, 4) == ([0, 3, 6, 9], [3, 3, 3, 6])
     assert bcr_p.get_process_rowblock_index(6, 3) == (6, 9)
+
+
+def test_bcr_parallel_schedule_known_answer():
+    """15 padded block rows over 2 ranks (rows [0, 7) and [7, 15)), worked by
+    hand from bcr_parallel.py:227-256 (a rank reduces the rows of i_elim in
+    its range) and :733-758 (it produces the rows of i_prod in its range)."""
+    from sinv_amd.algorithms import bcr_p
+    red = bcr_p.reduction_schedule(15, 2)
+    assert [r[0] for r in red] == [[1, 3, 5, 7, 9, 11, 13], [3, 7, 11], [7]]
+    assert [r[1] for r in red] == [[(0, 2), (3, 6)], [(0, 0), (1, 2)], [None, (0, 0)]]
+    prod = bcr_p.production_schedule(15, 2)
+    assert [p[0] for p in prod] == [[3, 11], [1, 5, 9, 13], [0, 2, 4, 6, 8, 10, 12, 14]]
+    assert [p[1] for p in prod] == [[[0], [1]], [[0, 1], [2, 3]], [[0, 1, 2, 3], [4, 5, 6, 7]]]
+
+
+@pytest.mark.parametrize("nbp", [1, 3, 7, 15, 31, 63, 127])
+def test_bcr_parallel_schedule_covers_every_row_once(nbp):
+    """For every rank count, each level's reduced and produced rows are split
+    among the ranks without gap or overlap, so one level-batched launch over
+    all partitions does the ranks' combined work."""
+    from sinv_amd.algorithms import bcr_p
+    for P in range(1, min(nbp, 9) + 1):
+        bcr_p.check_schedule_covers(nbp, P)
+        for i_elim, per in bcr_p.reduction_schedule(nbp, P):
+            bounds = [bcr_p.get_process_rowblock_index(s, z)
+                      for s, z in zip(*bcr_p.divide_matrix(nbp, P))]
+            for (top, bottom), r in zip(bounds, per):
+                if r is not None:
+                    assert all(top <= i_elim[q] < bottom for q in range(r[0], r[1] + 1))
+
+
+def test_bcr_parallel_schedule_check_catches_a_gap():
+    """The coverage check fails when a level's row is left to no partition
+    (a planted mistake: one rank's range dropped)."""
+    from sinv_amd.algorithms import bcr_p
+    orig = bcr_p.reduction_schedule
+
+    def dropped(nbp, P):
+        out = orig(nbp, P)
+        i_elim, per = out[0]
+        return [(i_elim, [None] + per[1:])] + out[1:]
+
+    bcr_p.reduction_schedule = dropped
+    try:
+        with pytest.raises(AssertionError):
+            bcr_p.check_schedule_covers(15, 2)
+    finally:
+        bcr_p.reduction_schedule = orig
