@@ -132,6 +132,20 @@ def test_zinv_singular_raises(built_lib):
     assert ei.value.instance == 1
 
 
+@pytest.mark.parametrize("n,zc,batch", [(64, 40, 4), (37, 20, 4), (64, 63, 600)])
+def test_zinv_singular_late_pivot(built_lib, n, zc, batch):
+    """A zero column stays exactly zero through the earlier steps, so the leaf
+    meets an exactly zero pivot at step zc: the inverse raises, naming the
+    instance (both panel widths: 4 and 600 blocks per launch)."""
+    from sinv_amd import core
+    rng = np.random.default_rng(3)
+    A = _rand(rng, batch, n, n)
+    A[batch - 1, :, zc] = 0.0
+    with pytest.raises(np.linalg.LinAlgError) as ei:
+        core.zinv_batched(A)
+    assert ei.value.instance == batch - 1
+
+
 def test_zinv_nan_input_propagates(built_lib):
     """NaN entries propagate to NaN results without a fault (no comparable
     pivot candidate keeps the current row)."""
