@@ -599,12 +599,17 @@ def bench_variants(args):
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
+        # one more solve under the launch profiler, inside a roctx range that a
+        # rocprofv3 --marker-trace run can select (sinv_variant:<method>)
+        torch.cuda.nvtx.range_push(f"sinv_variant:{name}")
         with _native.LaunchProfiler() as prof:
             fn()
             torch.cuda.synchronize()
+        torch.cuda.nvtx.range_pop()
         fl = sum(v[2] for v in prof.stats.values())
+        kt = {k: dict(ms=round(v[0], 3), launches=int(v[1])) for k, v in prof.stats.items() if v[1]}
         res[name] = dict(ms=round(ms, 3), executed_tflops=round(fl / (ms / 1e3) / 1e12, 3),
-                         rel_err_vs_rgf=float(np.sqrt(num / den)))
+                         rel_err_vs_rgf=float(np.sqrt(num / den)), kernel_time=kt)
         log(f"[bench] {name}: {res[name]}")
     rgf_fl = configs.rgf_flops(N, b)
     head = res["bcr_serial"] if res["bcr_serial"]["ms"] < res["rgf"]["ms"] else res["rgf"]

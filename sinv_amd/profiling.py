@@ -62,8 +62,8 @@ def _ranges(d, name):
         return []
     spans = []
     for r in csv.DictReader(open(f)):
-        txt = " ".join(str(v) for v in r.values())
-        if name in txt:
+        # exact match: "sinv_variant:rgf" must not select "sinv_variant:rgf2sided"
+        if any(str(v).strip() == name for v in r.values()):
             try:
                 spans.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
             except (KeyError, ValueError):
