@@ -245,13 +245,29 @@ int sinv_pdiv_batched(const sinv_pdiv_desc* d, void* work, size_t work_bytes, vo
  * Batched in-place complex128 block inverse (kernel K1).
  * Replaces the numpy.linalg.inv calls of the hot path (rgf.py:44,48;
  * bcr_serial.py:121,148,249,302,379).  Blocks with n <= 64 use Gauss-Jordan
- * with partial pivoting (the pivoting of LAPACK ?gesv); larger blocks use a
- * 2x2 Schur-complement recursion down to such leaves.
+ * with partial pivoting on |re| + |im| (LAPACK ?gesv's modulus); larger blocks
+ * use a 2x2 Schur-complement recursion down to such leaves.
  * Matrix z is at A + z*strideA (complex elements), leading dimension lda.
  * ------------------------------------------------------------------------ */
 size_t sinv_zinv_workspace_bytes(int n, int batch);
 int sinv_zinv_batched(void* A, int n, int lda, long long strideA, int batch, void* work,
                       size_t work_bytes, unsigned long long* info, void* stream);
+
+/* Leaf kernel selection (no counterpart in the reference; for tests and
+ * measurements).  SINV_LEAF_AUTO: the panel kernel, 16 waves of 4 columns for
+ * launches of fewer than 512 blocks, else 8 waves of 8 columns (pivot: the
+ * earliest row within 2^-15 of the column maximum).  SINV_LEAF_ROWS8: one
+ * barrier per elimination step, pivot = LAPACK izamax's exact first maximum.
+ * SINV_LEAF_LDS: the LDS-resident kernel (exact first maximum).  Process-wide,
+ * unsynchronised: set it before launching the work it applies to.  The
+ * environment variable SINV_LEAF_KERNEL (panel16, panel8, rows8, lds) sets the
+ * initial choice. */
+#define SINV_LEAF_AUTO 1
+#define SINV_LEAF_LDS 0
+#define SINV_LEAF_ROWS8 2
+#define SINV_LEAF_PANEL8 3
+#define SINV_LEAF_PANEL16 4
+int sinv_leaf_force_variant(int variant);
 
 /* ------------------------------------------------------------------------
  * Batched complex GEMM (kernel K2):

@@ -155,6 +155,8 @@ def lib():
     L.sinv_gemm_config_tile.argtypes = [_i, ctypes.POINTER(_i), ctypes.POINTER(_i)]
     L.sinv_gemm_force_config.restype = _i
     L.sinv_gemm_force_config.argtypes = [_i]
+    L.sinv_leaf_force_variant.restype = _i
+    L.sinv_leaf_force_variant.argtypes = [_i]
     L.sinv_gemm_tune_set.restype = _i
     L.sinv_gemm_tune_set.argtypes = [_i, _i, _i, _ll, _i, _i, _i, _i, _i]
     L.sinv_gemm_tune_clear.restype = _i
@@ -210,6 +212,15 @@ def check_info(info_tensor, what="inverse"):
 
 
 PROF_KINDS = ("gemm_f64", "gemm_f32", "inv_leaf", "other")
+
+# leaf inverse kernels (include/sinv.h SINV_LEAF_*)
+LEAF_VARIANTS = {"auto": 1, "lds": 0, "rows8": 2, "panel8": 3, "panel16": 4}
+
+
+def force_leaf_variant(name):
+    """Select the leaf inverse kernel for later launches ("auto" restores the
+    default choice)."""
+    check(lib().sinv_leaf_force_variant(LEAF_VARIANTS[name]), "leaf_force_variant")
 
 
 class LaunchProfiler:

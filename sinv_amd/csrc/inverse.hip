@@ -551,20 +551,22 @@ __global__ __launch_bounds__(64 * W) void inv_leaf_kernel_panel(cplx* A, int n, 
 // Leaf kernel choice: the panel kernel, with 16 waves of 4 columns when the
 // launch leaves CUs idle (fewer than two matrices per CU: the per-step
 // latency of the factoring wave is what counts) and 8 waves of 8 columns
-// otherwise (fewer, cheaper barriers; workgroups share CUs).  SINV_LEAF_KERNEL
-// forces one ("panel16", "panel8", "rows8": the register kernel with one
-// barrier per step, "lds": the LDS-resident kernel).  Read once per process.
+// otherwise (fewer, cheaper barriers; workgroups share CUs).
+// sinv_leaf_force_variant, or initially SINV_LEAF_KERNEL ("panel16",
+// "panel8", "rows8": the register kernel with one barrier per step, "lds":
+// the LDS-resident kernel), forces one.
+int g_leaf_variant = -1;  // SINV_LEAF_* value; -1 until first read
 int leaf_variant() {
-  static int v = -1;
-  if (v < 0) {
+  if (g_leaf_variant < 0) {
     const char* e = getenv("SINV_LEAF_KERNEL");
-    v = 1;
-    if (e != nullptr && strcmp(e, "lds") == 0) v = 0;
-    if (e != nullptr && strcmp(e, "rows8") == 0) v = 2;
-    if (e != nullptr && strcmp(e, "panel8") == 0) v = 3;
-    if (e != nullptr && strcmp(e, "panel16") == 0) v = 4;
+    int v = SINV_LEAF_AUTO;
+    if (e != nullptr && strcmp(e, "lds") == 0) v = SINV_LEAF_LDS;
+    if (e != nullptr && strcmp(e, "rows8") == 0) v = SINV_LEAF_ROWS8;
+    if (e != nullptr && strcmp(e, "panel8") == 0) v = SINV_LEAF_PANEL8;
+    if (e != nullptr && strcmp(e, "panel16") == 0) v = SINV_LEAF_PANEL16;
+    g_leaf_variant = v;
   }
-  return v;
+  return g_leaf_variant;
 }
 constexpr int kLeafPanel8MinGrid = 512;  // 2 x the 256 CUs of MI355X
 
@@ -596,17 +598,18 @@ hipError_t launch_inverse(cplx* A, int n, int lda, long long s_in, long long s_o
     const int pid = prof_open(s);
     const dim3 grid((unsigned)(inner * outer));
     int v = leaf_variant();
-    if (v == 1) v = grid.x >= (unsigned)kLeafPanel8MinGrid ? 3 : 4;
+    if (v == SINV_LEAF_AUTO)
+      v = grid.x >= (unsigned)kLeafPanel8MinGrid ? SINV_LEAF_PANEL8 : SINV_LEAF_PANEL16;
     switch (v) {
-      case 0:
+      case SINV_LEAF_LDS:
         hipLaunchKernelGGL(inv_leaf_kernel, grid, dim3(256), 0, s, A, n, lda, s_in, s_out, inner,
                            info, tag);
         break;
-      case 2:
+      case SINV_LEAF_ROWS8:
         hipLaunchKernelGGL(inv_leaf_kernel_rows<8>, grid, dim3(512), 0, s, A, n, lda, s_in,
                            s_out, inner, info, tag);
         break;
-      case 3:
+      case SINV_LEAF_PANEL8:
         hipLaunchKernelGGL(inv_leaf_kernel_panel<8>, grid, dim3(512), 0, s, A, n, lda, s_in,
                            s_out, inner, info, tag);
         break;
@@ -693,5 +696,16 @@ extern "C" int sinv_zinv_batched(void* A, int n, int lda, long long strideA, int
   hipStream_t s = (hipStream_t)stream;
   if (info != nullptr) SINV_HIP_TRY(launch_fill_u64(info, 1, SINV_INFO_OK, s));
   SINV_HIP_TRY(launch_inverse((cplx*)A, n, lda, strideA, 0, batch, 1, (cplx*)work, info, 0, s));
+  return SINV_OK;
+}
+
+extern "C" int sinv_leaf_force_variant(int variant) {
+  using namespace sinv;
+  if (variant != SINV_LEAF_AUTO && variant != SINV_LEAF_LDS && variant != SINV_LEAF_ROWS8 &&
+      variant != SINV_LEAF_PANEL8 && variant != SINV_LEAF_PANEL16) {
+    set_error("sinv_leaf_force_variant: unknown variant %d", variant);
+    return SINV_E_INVAL;
+  }
+  g_leaf_variant = variant;
   return SINV_OK;
 }

@@ -123,6 +123,41 @@ def test_zinv_near_tie_pivot(built_lib):
         assert err < 1e-15 * np.linalg.cond(A[z]) * n, (z, err)
 
 
+@pytest.mark.parametrize("variant", ["panel16", "panel8", "rows8", "lds"])
+def test_zinv_every_leaf_kernel(built_lib, variant):
+    """Every selectable leaf kernel inverts random, zero-leading-entry and
+    permutation blocks to numpy's result and flags a singular one; rows8 and
+    lds take LAPACK's exact first maximum as pivot, the panel kernels the
+    earliest row within 2^-15 of it."""
+    from sinv_amd import _native as nat, core
+    print(f"leaf variant {variant}", flush=True)
+    nat.force_leaf_variant(variant)
+    try:
+        for n in (2, 5, 33, 64):  # n = 1 with a zero entry would be singular
+            rng = np.random.default_rng(200 + n)
+            A = _rand(rng, 3, n, n)
+            A[1, 0, 0] = 0.0
+            A[2] = np.roll(np.eye(n), 1, axis=0) * (1 + 1j)
+            X = core.zinv_batched(A).cpu().numpy()
+            ref = np.linalg.inv(A)
+            for z in range(3):
+                err = np.linalg.norm(X[z] - ref[z]) / np.linalg.norm(ref[z])
+                assert err < 1e-15 * np.linalg.cond(A[z]) * max(n, 2), (n, z, err)
+        S = _rand(np.random.default_rng(9), 2, 20, 20)
+        S[1, :, 7] = 0.0
+        with pytest.raises(np.linalg.LinAlgError) as ei:
+            core.zinv_batched(S)
+        assert ei.value.instance == 1
+    finally:
+        nat.force_leaf_variant("auto")
+
+
+def test_leaf_variant_rejects_unknown(built_lib):
+    from sinv_amd import _native as nat
+    assert built_lib.sinv_leaf_force_variant(7) == nat.SINV_E_INVAL
+    assert built_lib.sinv_leaf_force_variant(nat.LEAF_VARIANTS["auto"]) == 0
+
+
 def test_zinv_singular_raises(built_lib):
     from sinv_amd import core
     A = np.zeros((2, 8, 8), complex)
