@@ -1,0 +1,86 @@
+"""Parity at the benchmark configurations' full sizes, through properties that
+do not need a dense or CPU reference there:
+
+* the selected-block residual identities of G = A^-1 on block row i,
+  A_{i,i-1} G_{i-1,i} + A_ii G_ii + A_{i,i+1} G_{i+1,i} = I, and on block
+  column i, G_{i,i-1} A_{i-1,i} + G_ii A_ii + G_{i,i+1} A_{i+1,i} = I;
+* RGF and BCR, two different eliminations, give the same blocks;
+* for A(E) = (E + i eta) I - H with H Hermitian and eta > 0 the density of
+  states -Im tr G_ii / pi is non-negative for every block.
+
+Sizes: the single large instance (128 blocks of 512, diagonally dominant
+generator) and the batched-energies workload's largest block size (64
+blocks of 384, Hermitian H, eta = 1e-3) at 8 energies.  The residuals are
+formed on the GPU with torch (complex128 products), so nothing of the 1.6 GB
+result leaves the device."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _residuals(d, u, l, Gd, Gu, Gl):
+    """Largest ||(A G)_ii - I||_F / sqrt(b) and ||(G A)_ii - I||_F / sqrt(b)
+    over instances and block rows; inputs (B, N, b, b) / (B, N-1, b, b)."""
+    import torch
+    b = d.shape[-1]
+    eye = torch.eye(b, dtype=d.dtype, device=d.device)
+    R = torch.matmul(d, Gd)
+    R[:, 1:] += torch.matmul(l, Gu)    # A_{i,i-1} G_{i-1,i}
+    R[:, :-1] += torch.matmul(u, Gl)   # A_{i,i+1} G_{i+1,i}
+    C = torch.matmul(Gd, d)
+    C[:, 1:] += torch.matmul(Gl, u)    # G_{i,i-1} A_{i-1,i}
+    C[:, :-1] += torch.matmul(Gu, l)   # G_{i,i+1} A_{i+1,i}
+    rn = torch.linalg.matrix_norm(R - eye).max().item() / np.sqrt(b)
+    cn = torch.linalg.matrix_norm(C - eye).max().item() / np.sqrt(b)
+    return rn, cn
+
+
+def _rel(x, y):
+    import torch
+    num = sum(float(torch.linalg.vector_norm(a - c) ** 2) for a, c in zip(x, y))
+    den = sum(float(torch.linalg.vector_norm(c) ** 2) for c in y)
+    return np.sqrt(num / den)
+
+
+def test_single_large_instance_rgf_and_bcr(built_lib):
+    """128 blocks of 512 (n = 65536), batch 1: both selected inversions satisfy
+    the residual identities and agree with each other."""
+    import torch
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    print("full size: 128 blocks of 512, rgf + bcr", flush=True)
+    d, u, l = (core.to_dev(x)[None] for x in configs.dominant_blocks(128, 512, seed=2))
+    G_rgf = core.rgf_batched(d, u, l)
+    G_bcr = core.bcr_batched(d, u, l)
+    torch.cuda.synchronize()
+    for G in (G_rgf, G_bcr):
+        rn, cn = _residuals(d, u, l, *G)
+        assert rn < 1e-12 and cn < 1e-12, (rn, cn)
+    assert _rel(G_bcr, G_rgf) < 1e-12
+
+
+def test_energies_full_blocks_residuals_and_dos(built_lib):
+    """64 blocks of 384, A(E) = (E + 1e-3 i) I - H at 8 energies in [-2, 2]
+    (inside the band, where A is least well conditioned): the residual
+    identities hold to the conditioning and the DOS of every block is
+    non-negative."""
+    import torch
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    print("full size: 64 blocks of 384, 8 energies", flush=True)
+    eta = 1e-3
+    H = configs.hermitian_H(64, 384, seed=12)
+    E = configs.energies(8, seed=22)
+    Gd, Gu, Gl = core.rgf_energies(*H, E, eta)
+    Hd, Hu, Hl = (core.to_dev(x) for x in H)
+    sh = torch.as_tensor(E + 1j * eta, dtype=torch.complex128, device=Hd.device)
+    eye = torch.eye(384, dtype=torch.complex128, device=Hd.device)
+    d = sh[:, None, None, None] * eye - Hd[None]
+    u = (-Hu)[None].expand(len(E), -1, -1, -1).contiguous()
+    l = (-Hl)[None].expand(len(E), -1, -1, -1).contiguous()
+    rn, cn = _residuals(d, u, l, Gd, Gu, Gl)
+    assert rn < 1e-9 and cn < 1e-9, (rn, cn)
+    dos = -torch.diagonal(Gd, dim1=-2, dim2=-1).imag.sum(-1) / np.pi   # (energies, blocks)
+    assert float(dos.min()) >= 0.0, float(dos.min())
