@@ -84,3 +84,48 @@ def test_energies_full_blocks_residuals_and_dos(built_lib):
     assert rn < 1e-9 and cn < 1e-9, (rn, cn)
     dos = -torch.diagonal(Gd, dim1=-2, dim2=-1).imag.sum(-1) / np.pi   # (energies, blocks)
     assert float(dos.min()) >= 0.0, float(dos.min())
+
+
+def test_adjoint_full_blocks_directional_derivative(built_lib):
+    """The forward+adjoint workload's instance (96 blocks, sizes 256 / 416 mixed
+    inside the instance, g checkpointed every 8 blocks) at 4 energies, eta =
+    1e-3, L = -(1/pi) sum_E sum_i Im tr G_ii: the adjoint gradient's
+    directional derivative Re <dL/dH, D> along a random block-tridiagonal D
+    equals the central difference of L along D (h = 1e-6)."""
+    import torch
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    print("full size: 96 ragged blocks (256/416), 4 energies, adjoint", flush=True)
+    rng = np.random.default_rng(31)
+    bs = [int(x) for x in rng.choice([256, 416], size=96)]
+    H = configs.hermitian_H_ragged(bs, seed=32)
+    E = configs.energies(4, seed=33) + 1e-3j
+    dev = torch.device("cuda")
+    Hd = [core.to_dev(x) for x in H[0]]
+    Hu = [core.to_dev(x) for x in H[1]]
+    Hl = [core.to_dev(x) for x in H[2]]
+    gd = [(-1j / np.pi) * torch.eye(b, dtype=torch.complex128, device=dev) for b in bs]
+    zu = [torch.zeros_like(x) for x in Hu]
+    zl = [torch.zeros_like(x) for x in Hl]
+    Ad, Au, Al, Gd = core.rgf_adjoint_ragged(Hd, Hu, Hl, gd, zu, zl, shift=E, dscale=-1.0,
+                                             oscale=-1.0, checkpoint_every=8,
+                                             return_g_diag=True)
+    grad = [-a.sum(0) for a in Ad + Au + Al]   # A = (E + i eta) I - H
+    drng = np.random.default_rng(5)
+    D = [torch.as_tensor(drng.standard_normal(x.shape) + 1j * drng.standard_normal(x.shape),
+                         device=dev) for x in Hd + Hu + Hl]
+    ddir = sum(float((torch.conj(g) * d).real.sum()) for g, d in zip(grad, D))
+
+    def loss(t):
+        nd, nu = len(Hd), len(Hu)
+        X = [h + t * d for h, d in zip(Hd + Hu + Hl, D)]
+        G = core.rgf_ragged(X[:nd], X[nd:nd + nu], X[nd + nu:], shift=E, dscale=-1.0,
+                            oscale=-1.0, save_off_diag=False)[0]
+        return sum(-float(torch.diagonal(g, dim1=-2, dim2=-1).imag.sum()) / np.pi for g in G)
+
+    L0 = sum(-float(torch.diagonal(g, dim1=-2, dim2=-1).imag.sum()) / np.pi for g in Gd)
+    assert abs(loss(0.0) - L0) < 1e-11 * abs(L0)
+    h = 1e-6
+    fd = (loss(h) - loss(-h)) / (2 * h)
+    print(f"directional derivative: adjoint {ddir:.12e}, central difference {fd:.12e}", flush=True)
+    assert abs(fd - ddir) < 1e-5 * abs(ddir), (fd, ddir)
