@@ -254,19 +254,23 @@ int sinv_zinv_batched(void* A, int n, int lda, long long strideA, int batch, voi
                       size_t work_bytes, unsigned long long* info, void* stream);
 
 /* Leaf kernel selection (no counterpart in the reference; for tests and
- * measurements).  SINV_LEAF_AUTO: the panel kernel, 16 waves of 4 columns for
- * launches of fewer than 512 blocks, else 8 waves of 8 columns (pivot: the
- * earliest row within 2^-15 of the column maximum).  SINV_LEAF_ROWS8: one
- * barrier per elimination step, pivot = LAPACK izamax's exact first maximum.
- * SINV_LEAF_LDS: the LDS-resident kernel (exact first maximum).  Process-wide,
- * unsynchronised: set it before launching the work it applies to.  The
- * environment variable SINV_LEAF_KERNEL (panel16, panel8, rows8, lds) sets the
+ * measurements).  SINV_LEAF_AUTO: the panel kernel with flag hand-off
+ * between waves, 16 waves of 4 columns for launches of fewer than 512
+ * blocks, else 8 waves of 8 columns (pivot: the earliest row within 2^-15 of
+ * the column maximum); SINV_LEAF_PANEL16 / PANEL8: the same with one barrier
+ * per panel.  SINV_LEAF_ROWS8: one barrier per elimination step, pivot =
+ * LAPACK izamax's exact first maximum.  SINV_LEAF_LDS: the LDS-resident
+ * kernel (exact first maximum).  Process-wide, unsynchronised: set it before
+ * launching the work it applies to.  The environment variable
+ * SINV_LEAF_KERNEL (panel16f, panel8f, panel16, panel8, rows8, lds) sets the
  * initial choice. */
 #define SINV_LEAF_AUTO 1
 #define SINV_LEAF_LDS 0
 #define SINV_LEAF_ROWS8 2
 #define SINV_LEAF_PANEL8 3
 #define SINV_LEAF_PANEL16 4
+#define SINV_LEAF_PANEL16_FLAGS 5
+#define SINV_LEAF_PANEL8_FLAGS 6
 int sinv_leaf_force_variant(int variant);
 
 /* ------------------------------------------------------------------------

@@ -214,7 +214,8 @@ def check_info(info_tensor, what="inverse"):
 PROF_KINDS = ("gemm_f64", "gemm_f32", "inv_leaf", "other")
 
 # leaf inverse kernels (include/sinv.h SINV_LEAF_*)
-LEAF_VARIANTS = {"auto": 1, "lds": 0, "rows8": 2, "panel8": 3, "panel16": 4}
+LEAF_VARIANTS = {"auto": 1, "lds": 0, "rows8": 2, "panel8": 3, "panel16": 4, "panel16f": 5,
+                 "panel8f": 6}
 
 
 def force_leaf_variant(name):

@@ -473,15 +473,26 @@ __device__ __forceinline__ void leaf_apply_panel(cplx (&a)[R], int kb, int n, in
   }
 }
 
-template <int W>
+// Panel hand-off: with FLAGS every panel has its own LDS slot and a ready
+// flag that its owner sets (release) once the slot is written; the other
+// waves wait for that flag (acquire) instead of a workgroup barrier, so no
+// wave waits for a slower one it does not depend on.  The wait is bounded
+// (kLeafSpinLimit polls); a wave that runs out stops applying and the block
+// is reported through info like a singular one.  Without FLAGS two slots
+// alternate and one barrier per panel separates writers from readers.
+constexpr int kLeafSpinLimit = 1 << 22;
+
+template <int W, bool FLAGS = false>
 __global__ __launch_bounds__(64 * W) void inv_leaf_kernel_panel(cplx* A, int n, int lda,
                                                                 long long s_in, long long s_out,
                                                                 int inner,
                                                                 unsigned long long* info,
                                                                 int tag) {
   constexpr int R = 64 / W;
-  __shared__ cplx gbuf[2][R][64];  // per panel parity: multiplier vector of each step
-  __shared__ int pbuf[2][R];       // pivot row of each step
+  constexpr int NSLOT = FLAGS ? W : 2;
+  __shared__ cplx gbuf[NSLOT][R][64];  // multiplier vector of each step of a panel
+  __shared__ int pbuf[NSLOT][R];       // pivot row of each step
+  __shared__ int ready[W];             // FLAGS: panel published
   __shared__ int pivpos[64];
   __shared__ int invp[64];
 
@@ -506,9 +517,14 @@ __global__ __launch_bounds__(64 * W) void inv_leaf_kernel_panel(cplx* A, int n, 
 
   int pos = lane;  // position of this row in the swapped row order
   bool singular = false;
+  bool stuck = false;
   const int npanels = (n + R - 1) / R;
+  if constexpr (FLAGS) {
+    if (threadIdx.x < W) ready[threadIdx.x] = 0;
+    __syncthreads();
+  }
   for (int kb = 0; kb < npanels; ++kb) {
-    const int buf = kb & 1;
+    const int buf = FLAGS ? kb : (kb & 1);
     const bool full = kb * R + R <= n;
     if (w == kb) {
       __builtin_amdgcn_s_setprio(3);
@@ -516,9 +532,26 @@ __global__ __launch_bounds__(64 * W) void inv_leaf_kernel_panel(cplx* A, int n, 
         leaf_factor_panel<R, true>(a, kb, n, lane, pos, singular, gbuf[buf], pbuf[buf], pivpos);
       else
         leaf_factor_panel<R, false>(a, kb, n, lane, pos, singular, gbuf[buf], pbuf[buf], pivpos);
+      if constexpr (FLAGS) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == 0) __hip_atomic_store(&ready[kb], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
     }
-    __syncthreads();
+    if constexpr (!FLAGS) __syncthreads();
     if (w != kb) {
+      if constexpr (FLAGS) {
+        if (stuck) continue;
+        int polls = 0;
+        while (__hip_atomic_load(&ready[kb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) {
+          if (++polls > kLeafSpinLimit) break;
+          __builtin_amdgcn_s_sleep(1);
+        }
+        if (polls > kLeafSpinLimit) {
+          stuck = true;
+          continue;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      }
       // the next panel's owner applies first: it is on the critical path
       if (w == kb + 1)
         __builtin_amdgcn_s_setprio(2);
@@ -534,7 +567,7 @@ __global__ __launch_bounds__(64 * W) void inv_leaf_kernel_panel(cplx* A, int n, 
   // inv(A) = inv(P A) P: column c of the eliminated matrix is column invp[c]
   // of the result, where invp inverts the final row positions
   if (w == 0 && lane < n) invp[pos] = lane;
-  if (lane == 0 && singular && info != nullptr)
+  if (lane == 0 && (singular || stuck) && info != nullptr)
     atomicMin(info, ((unsigned long long)(unsigned)tag << 32) | (unsigned long long)(unsigned)z);
   __syncthreads();
   if (lane < n) {
@@ -548,13 +581,14 @@ __global__ __launch_bounds__(64 * W) void inv_leaf_kernel_panel(cplx* A, int n, 
   }
 }
 
-// Leaf kernel choice: the panel kernel, with 16 waves of 4 columns when the
-// launch leaves CUs idle (fewer than two matrices per CU: the per-step
-// latency of the factoring wave is what counts) and 8 waves of 8 columns
-// otherwise (fewer, cheaper barriers; workgroups share CUs).
-// sinv_leaf_force_variant, or initially SINV_LEAF_KERNEL ("panel16",
-// "panel8", "rows8": the register kernel with one barrier per step, "lds":
-// the LDS-resident kernel), forces one.
+// Leaf kernel choice: the panel kernel with flag hand-off, 16 waves of 4
+// columns when the launch leaves CUs idle (fewer than two matrices per CU:
+// the per-step latency of the factoring wave is what counts) and 8 waves of
+// 8 columns otherwise (fewer hand-offs; workgroups share CUs).
+// sinv_leaf_force_variant, or initially SINV_LEAF_KERNEL ("panel16f",
+// "panel8f", "panel16" / "panel8": one barrier per panel, "rows8": the
+// register kernel with one barrier per step, "lds": the LDS-resident
+// kernel), forces one.
 int g_leaf_variant = -1;  // SINV_LEAF_* value; -1 until first read
 int leaf_variant() {
   if (g_leaf_variant < 0) {
@@ -564,6 +598,8 @@ int leaf_variant() {
     if (e != nullptr && strcmp(e, "rows8") == 0) v = SINV_LEAF_ROWS8;
     if (e != nullptr && strcmp(e, "panel8") == 0) v = SINV_LEAF_PANEL8;
     if (e != nullptr && strcmp(e, "panel16") == 0) v = SINV_LEAF_PANEL16;
+    if (e != nullptr && strcmp(e, "panel16f") == 0) v = SINV_LEAF_PANEL16_FLAGS;
+    if (e != nullptr && strcmp(e, "panel8f") == 0) v = SINV_LEAF_PANEL8_FLAGS;
     g_leaf_variant = v;
   }
   return g_leaf_variant;
@@ -599,7 +635,8 @@ hipError_t launch_inverse(cplx* A, int n, int lda, long long s_in, long long s_o
     const dim3 grid((unsigned)(inner * outer));
     int v = leaf_variant();
     if (v == SINV_LEAF_AUTO)
-      v = grid.x >= (unsigned)kLeafPanel8MinGrid ? SINV_LEAF_PANEL8 : SINV_LEAF_PANEL16;
+      v = grid.x >= (unsigned)kLeafPanel8MinGrid ? SINV_LEAF_PANEL8_FLAGS
+                                                 : SINV_LEAF_PANEL16_FLAGS;
     switch (v) {
       case SINV_LEAF_LDS:
         hipLaunchKernelGGL(inv_leaf_kernel, grid, dim3(256), 0, s, A, n, lda, s_in, s_out, inner,
@@ -612,6 +649,14 @@ hipError_t launch_inverse(cplx* A, int n, int lda, long long s_in, long long s_o
       case SINV_LEAF_PANEL8:
         hipLaunchKernelGGL(inv_leaf_kernel_panel<8>, grid, dim3(512), 0, s, A, n, lda, s_in,
                            s_out, inner, info, tag);
+        break;
+      case SINV_LEAF_PANEL16_FLAGS:
+        hipLaunchKernelGGL((inv_leaf_kernel_panel<16, true>), grid, dim3(1024), 0, s, A, n, lda,
+                           s_in, s_out, inner, info, tag);
+        break;
+      case SINV_LEAF_PANEL8_FLAGS:
+        hipLaunchKernelGGL((inv_leaf_kernel_panel<8, true>), grid, dim3(512), 0, s, A, n, lda,
+                           s_in, s_out, inner, info, tag);
         break;
       default:
         hipLaunchKernelGGL(inv_leaf_kernel_panel<16>, grid, dim3(1024), 0, s, A, n, lda, s_in,
@@ -702,7 +747,8 @@ extern "C" int sinv_zinv_batched(void* A, int n, int lda, long long strideA, int
 extern "C" int sinv_leaf_force_variant(int variant) {
   using namespace sinv;
   if (variant != SINV_LEAF_AUTO && variant != SINV_LEAF_LDS && variant != SINV_LEAF_ROWS8 &&
-      variant != SINV_LEAF_PANEL8 && variant != SINV_LEAF_PANEL16) {
+      variant != SINV_LEAF_PANEL8 && variant != SINV_LEAF_PANEL16 &&
+      variant != SINV_LEAF_PANEL16_FLAGS && variant != SINV_LEAF_PANEL8_FLAGS) {
     set_error("sinv_leaf_force_variant: unknown variant %d", variant);
     return SINV_E_INVAL;
   }

@@ -123,7 +123,7 @@ def test_zinv_near_tie_pivot(built_lib):
         assert err < 1e-15 * np.linalg.cond(A[z]) * n, (z, err)
 
 
-@pytest.mark.parametrize("variant", ["panel16", "panel8", "rows8", "lds"])
+@pytest.mark.parametrize("variant", ["panel16", "panel8", "panel16f", "panel8f", "rows8", "lds"])
 def test_zinv_every_leaf_kernel(built_lib, variant):
     """Every selectable leaf kernel inverts random, zero-leading-entry and
     permutation blocks to numpy's result and flags a singular one; rows8 and
