@@ -272,6 +272,11 @@ int sinv_zinv_batched(void* A, int n, int lda, long long strideA, int batch, voi
 #define SINV_LEAF_PANEL16_FLAGS 5
 #define SINV_LEAF_PANEL8_FLAGS 6
 int sinv_leaf_force_variant(int variant);
+/* Bound on a leaf wave's wait for a panel hand-off (no counterpart in the
+ * reference; for failure-handling tests).  polls = -1 restores the default
+ * (2^22); a wave that exhausts it stops and its block reports as singular
+ * through info, so the kernel always exits. */
+int sinv_leaf_spin_limit(int polls);
 
 /* ------------------------------------------------------------------------
  * Batched complex GEMM (kernel K2):

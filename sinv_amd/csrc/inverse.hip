@@ -477,17 +477,19 @@ __device__ __forceinline__ void leaf_apply_panel(cplx (&a)[R], int kb, int n, in
 // flag that its owner sets (release) once the slot is written; the other
 // waves wait for that flag (acquire) instead of a workgroup barrier, so no
 // wave waits for a slower one it does not depend on.  The wait is bounded
-// (kLeafSpinLimit polls); a wave that runs out stops applying and the block
-// is reported through info like a singular one.  Without FLAGS two slots
-// alternate and one barrier per panel separates writers from readers.
+// (spin_limit polls, kLeafSpinLimit unless a test lowers it); a wave that
+// runs out stops applying and the block is reported through info like a
+// singular one.  Without FLAGS two slots alternate and one barrier per panel
+// separates writers from readers.
 constexpr int kLeafSpinLimit = 1 << 22;
+int g_leaf_spin_limit = kLeafSpinLimit;
 
 template <int W, bool FLAGS = false>
 __global__ __launch_bounds__(64 * W) void inv_leaf_kernel_panel(cplx* A, int n, int lda,
                                                                 long long s_in, long long s_out,
                                                                 int inner,
                                                                 unsigned long long* info,
-                                                                int tag) {
+                                                                int tag, int spin_limit) {
   constexpr int R = 64 / W;
   constexpr int NSLOT = FLAGS ? W : 2;
   __shared__ cplx gbuf[NSLOT][R][64];  // multiplier vector of each step of a panel
@@ -543,10 +545,10 @@ __global__ __launch_bounds__(64 * W) void inv_leaf_kernel_panel(cplx* A, int n, 
         if (stuck) continue;
         int polls = 0;
         while (__hip_atomic_load(&ready[kb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) {
-          if (++polls > kLeafSpinLimit) break;
+          if (++polls > spin_limit) break;
           __builtin_amdgcn_s_sleep(1);
         }
-        if (polls > kLeafSpinLimit) {
+        if (polls > spin_limit) {
           stuck = true;
           continue;
         }
@@ -648,19 +650,19 @@ hipError_t launch_inverse(cplx* A, int n, int lda, long long s_in, long long s_o
         break;
       case SINV_LEAF_PANEL8:
         hipLaunchKernelGGL(inv_leaf_kernel_panel<8>, grid, dim3(512), 0, s, A, n, lda, s_in,
-                           s_out, inner, info, tag);
+                           s_out, inner, info, tag, g_leaf_spin_limit);
         break;
       case SINV_LEAF_PANEL16_FLAGS:
         hipLaunchKernelGGL((inv_leaf_kernel_panel<16, true>), grid, dim3(1024), 0, s, A, n, lda,
-                           s_in, s_out, inner, info, tag);
+                           s_in, s_out, inner, info, tag, g_leaf_spin_limit);
         break;
       case SINV_LEAF_PANEL8_FLAGS:
         hipLaunchKernelGGL((inv_leaf_kernel_panel<8, true>), grid, dim3(512), 0, s, A, n, lda,
-                           s_in, s_out, inner, info, tag);
+                           s_in, s_out, inner, info, tag, g_leaf_spin_limit);
         break;
       default:
         hipLaunchKernelGGL(inv_leaf_kernel_panel<16>, grid, dim3(1024), 0, s, A, n, lda, s_in,
-                           s_out, inner, info, tag);
+                           s_out, inner, info, tag, g_leaf_spin_limit);
     }
     prof_close(pid, s, PROF_INV_LEAF, 8.0 * n * (double)n * n * inner * (double)outer,
                2.0 * sizeof(cplx) * n * (double)n * inner * (double)outer);
@@ -753,5 +755,15 @@ extern "C" int sinv_leaf_force_variant(int variant) {
     return SINV_E_INVAL;
   }
   g_leaf_variant = variant;
+  return SINV_OK;
+}
+
+extern "C" int sinv_leaf_spin_limit(int polls) {
+  using namespace sinv;
+  if (polls < -1) {
+    set_error("sinv_leaf_spin_limit: polls must be >= 0, or -1 for the default");
+    return SINV_E_INVAL;
+  }
+  g_leaf_spin_limit = polls < 0 ? kLeafSpinLimit : polls;
   return SINV_OK;
 }

@@ -152,6 +152,26 @@ def test_zinv_every_leaf_kernel(built_lib, variant):
         nat.force_leaf_variant("auto")
 
 
+def test_leaf_handoff_wait_is_bounded(built_lib):
+    """Failure handling of the leaf's flag hand-off, forced from the host: with
+    the wait bound set to zero polls the waves that would wait give up at
+    once, the kernel still exits, and the blocks report as singular; with the
+    default bound restored the same blocks invert normally."""
+    from sinv_amd import _native as nat, core
+    rng = np.random.default_rng(17)
+    A = _rand(rng, 4, 64, 64) + 128 * np.eye(64)
+    nat.force_leaf_variant("panel16f")
+    try:
+        assert built_lib.sinv_leaf_spin_limit(0) == 0
+        with pytest.raises(np.linalg.LinAlgError):
+            core.zinv_batched(A)
+    finally:
+        assert built_lib.sinv_leaf_spin_limit(-1) == 0
+        nat.force_leaf_variant("auto")
+    X = core.zinv_batched(A).cpu().numpy()
+    assert np.abs(X - np.linalg.inv(A)).max() < 1e-13
+
+
 def test_leaf_variant_rejects_unknown(built_lib):
     from sinv_amd import _native as nat
     assert built_lib.sinv_leaf_force_variant(7) == nat.SINV_E_INVAL
