@@ -473,6 +473,41 @@ __device__ __forceinline__ void leaf_apply_panel(cplx (&a)[R], int kb, int n, in
   }
 }
 
+// A full panel's R steps applied column by column (each column takes the R
+// steps in order, so every element sees the same operations as above): the
+// first column is done after R updates, and the pivot search of the next
+// panel, which needs only that column, can start while the other columns
+// are still being updated.
+template <int R>
+__device__ __forceinline__ void leaf_apply_panel_cols(cplx (&a)[R], int kb, int lane, int& pos,
+                                                      const cplx (*gb)[64], const int* pb,
+                                                      const int* pivpos) {
+  int p[R];
+  cplx g[R];
+  double keep[R];
+#pragma unroll
+  for (int kr = 0; kr < R; ++kr) {
+    p[kr] = __builtin_amdgcn_readfirstlane(pb[kr]);
+    g[kr] = gb[kr][lane];
+    keep[kr] = lane == p[kr] ? 0.0 : 1.0;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+#pragma unroll
+    for (int kr = 0; kr < R; ++kr) {
+      const cplx P{readlane_f64(a[r].re, p[kr]), readlane_f64(a[r].im, p[kr])};
+      a[r] = cplx{keep[kr] * a[r].re - (g[kr].re * P.re - g[kr].im * P.im),
+                  keep[kr] * a[r].im - (g[kr].re * P.im + g[kr].im * P.re)};
+    }
+  }
+#pragma unroll
+  for (int kr = 0; kr < R; ++kr) {
+    const int k = kb * R + kr;
+    const int pp = __builtin_amdgcn_readfirstlane(pivpos[k]);
+    pos = pos == k ? pp : (lane == p[kr] ? k : pos);
+  }
+}
+
 // Panel hand-off: with FLAGS every panel has its own LDS slot and a ready
 // flag that its owner sets (release) once the slot is written; the other
 // waves wait for that flag (acquire) instead of a workgroup barrier, so no
@@ -525,10 +560,11 @@ __global__ __launch_bounds__(64 * W) void inv_leaf_kernel_panel(cplx* A, int n, 
     if (threadIdx.x < W) ready[threadIdx.x] = 0;
     __syncthreads();
   }
+  bool prefactored = false;  // FLAGS: this wave's panel was factored (and published) early
   for (int kb = 0; kb < npanels; ++kb) {
     const int buf = FLAGS ? kb : (kb & 1);
     const bool full = kb * R + R <= n;
-    if (w == kb) {
+    if (w == kb && !prefactored) {
       __builtin_amdgcn_s_setprio(3);
       if (full)
         leaf_factor_panel<R, true>(a, kb, n, lane, pos, singular, gbuf[buf], pbuf[buf], pivpos);
@@ -553,6 +589,20 @@ __global__ __launch_bounds__(64 * W) void inv_leaf_kernel_panel(cplx* A, int n, 
           continue;
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        // the next panel's owner, both panels full: apply this one column by
+        // column and factor its own right away, so that its first pivot
+        // search overlaps the rest of the apply; publish it now
+        if (w == kb + 1 && full && (kb + 1) * R + R <= n) {
+          __builtin_amdgcn_s_setprio(3);
+          leaf_apply_panel_cols<R>(a, kb, lane, pos, gbuf[buf], pbuf[buf], pivpos);
+          leaf_factor_panel<R, true>(a, kb + 1, n, lane, pos, singular, gbuf[kb + 1],
+                                     pbuf[kb + 1], pivpos);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          if (lane == 0)
+            __hip_atomic_store(&ready[kb + 1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          prefactored = true;
+          continue;
+        }
       }
       // the next panel's owner applies first: it is on the critical path
       if (w == kb + 1)

@@ -152,6 +152,26 @@ def test_zinv_every_leaf_kernel(built_lib, variant):
         nat.force_leaf_variant("auto")
 
 
+def test_panel_leaf_variants_agree_bitwise(built_lib):
+    """The panel kernels differ only in how panels are handed from wave to
+    wave (barrier or flags, apply order row- or column-wise, 8 or 16 waves):
+    every element receives the same update sequence, so all four give the
+    same bits, for full and ragged panels."""
+    from sinv_amd import _native as nat, core
+    rng = np.random.default_rng(23)
+    outs = {}
+    try:
+        for v in ("panel16", "panel16f", "panel8", "panel8f"):
+            nat.force_leaf_variant(v)
+            outs[v] = [core.zinv_batched(_rand(np.random.default_rng(n), 6, n, n)).cpu().numpy()
+                       for n in (64, 61, 37, 8)]
+    finally:
+        nat.force_leaf_variant("auto")
+    for v in ("panel16f", "panel8", "panel8f"):
+        for x, y in zip(outs[v], outs["panel16"]):
+            assert np.array_equal(x, y), v
+
+
 def test_leaf_handoff_wait_is_bounded(built_lib):
     """Failure handling of the leaf's flag hand-off, forced from the host: with
     the wait bound set to zero polls the waves that would wait give up at
