@@ -172,6 +172,25 @@ def test_panel_leaf_variants_agree_bitwise(built_lib):
             assert np.array_equal(x, y), v
 
 
+def test_leaf_every_size_both_widths(built_lib):
+    """Every leaf size n = 1 .. 64 (full and partial last panels for both
+    widths), random non-dominant blocks that need pivoting: numpy's inverse
+    to the conditioning, with 4 blocks (16-wave panels) and 600 blocks
+    (8-wave panels) per launch."""
+    from sinv_amd import core
+    for n in range(1, 65):
+        rng = np.random.default_rng(1000 + n)
+        A = _rand(rng, 600, n, n)
+        X_big = core.zinv_batched(A).cpu().numpy()
+        X_small = core.zinv_batched(A[:4]).cpu().numpy()
+        ref = np.linalg.inv(A[:4])
+        for z in range(4):
+            tol = 1e-15 * np.linalg.cond(A[z]) * max(n, 2) + 1e-300
+            rn = np.linalg.norm(ref[z])
+            assert np.linalg.norm(X_small[z] - ref[z]) / rn < tol, (n, z)
+            assert np.linalg.norm(X_big[z] - ref[z]) / rn < tol, (n, z)
+
+
 def test_leaf_handoff_wait_is_bounded(built_lib):
     """Failure handling of the leaf's flag hand-off, forced from the host: with
     the wait bound set to zero polls the waves that would wait give up at
