@@ -95,7 +95,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(energies shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+              "(energies would shard across GPUs with no communication, one process per GPU, "
+              "so the per-GPU figure is the one measured here)", file=sys.stderr)
         return 2
     import torch
     from sinv_amd import core, _native
