@@ -321,7 +321,11 @@ int sinv_scatter_blocks(const void* blocks, long long strideS, int count, const 
  * numpy's BLAS picks its own blocking).  Every complex GEMM the library issues
  * (sinv_zgemm_batched and the ones inside the RGF, BCR, PSR, adjoint and
  * inverse drivers) runs one of sinv_gemm_config_count() tile configurations;
- * sinv_gemm_config_tile gives its workgroup tile (bm x bn).  The config is
+ * sinv_gemm_config_tile gives its workgroup tile (bm x bn), and
+ * sinv_gemm_config_ksplit the number of k-slices its workgroup sums (1: one
+ * sequential pass over k).  Configs with the same k-split accumulate every
+ * output in the same order and give identical bits; configs with different
+ * k-splits agree to rounding (-SINV_E_INVAL for a bad id).  The config is
  * chosen per launch: a forced config if set (sinv_gemm_force_config, -1
  * clears), else the tuned entry of the exact shape (M, N, K, batch, opA, opB,
  * C0 present, precision) if one was set with sinv_gemm_tune_set, else a fill
@@ -332,6 +336,7 @@ int sinv_scatter_blocks(const void* blocks, long long strideS, int count, const 
  * ------------------------------------------------------------------------ */
 int sinv_gemm_config_count(void);
 int sinv_gemm_config_tile(int cfg, int* bm, int* bn);
+int sinv_gemm_config_ksplit(int cfg);
 int sinv_gemm_force_config(int cfg);
 int sinv_gemm_tune_set(int M, int N, int K, long long batch, int opA, int opB, int has_c0,
                        int precision, int cfg);

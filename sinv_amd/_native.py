@@ -153,6 +153,8 @@ def lib():
     L.sinv_gemm_config_count.argtypes = []
     L.sinv_gemm_config_tile.restype = _i
     L.sinv_gemm_config_tile.argtypes = [_i, ctypes.POINTER(_i), ctypes.POINTER(_i)]
+    L.sinv_gemm_config_ksplit.restype = _i
+    L.sinv_gemm_config_ksplit.argtypes = [_i]
     L.sinv_gemm_force_config.restype = _i
     L.sinv_gemm_force_config.argtypes = [_i]
     L.sinv_leaf_force_variant.restype = _i

@@ -1,10 +1,11 @@
 """GEMM tile-shape autotuner.
 
 Every complex GEMM of the library runs one of a few tile configurations
-(``configs()``: 64 x 64 with 4 waves, 32 x 32 with 4 or 1 waves, 128 x 64 and
-64 x 128 with 8 waves).  Which one is fastest depends on the shape and the
-batch: small or unbatched products need the small tiles to fill 256 CUs, large
-batched ones run best on the default tile.  The tuner
+(``configs()``: 64 x 64 with 4 waves, 32 x 32 with 4 or 1 waves, 64 x 32 and
+32 x 64, 128 x 64 and 64 x 128 with 8 waves, and 32 x 32 / 16 x 16 tiles whose
+waves split k among them).  Which one is fastest depends on the shape and the
+batch: small or unbatched products need the small and split-k tiles to fill
+256 CUs, large batched ones run best on the default tile.  The tuner
 
 1. runs a workload once under the launch profiler to collect the exact GEMM
    shapes it issues (M, N, K, batch, operand ops, C0 present, precision),
@@ -71,34 +72,54 @@ def _operands(key):
 
 
 def time_config(key, cfg, reps=3, operands=None):
-    """Average ms of one launch of shape `key` with tile configuration `cfg`."""
+    """Average ms of one launch of shape `key` with tile configuration `cfg`.
+
+    The launches are timed as the library's drivers issue them: back to back
+    on one stream with the host far ahead.  Short kernels are therefore timed
+    from a captured HIP graph of at least ``reps`` launches (enough for about
+    0.2 ms of GPU time), so that the host's per-launch cost, several times a
+    small kernel's duration when launches come one by one from Python, does
+    not decide the choice."""
     M, N, K, batch, opA, opB, has_c0, prec = key
     A, B, C, C0 = operands if operands is not None else _operands(key)
     L = nat.lib()
     one = nat.C128(1.0, 0.0)
     beta = nat.C128(1.0, 0.0) if has_c0 else nat.C128(0.0, 0.0)
-    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
     def launch():
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         nat.check(L.sinv_zgemm_batched(
             M, N, K, opA, opB, one, p(A), A.shape[2], A.shape[1] * A.shape[2], p(B), B.shape[2],
             B.shape[1] * B.shape[2], beta, p(C0), N, M * N, p(C), N, M * N, None, batch, prec,
             stream), "zgemm")
 
-    nat.check(L.sinv_gemm_force_config(cfg), "force_config")
-    try:
-        launch()
+    def elapsed(fn):
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        for _ in range(reps):
-            launch()
+        fn()
         e1.record()
         torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    nat.check(L.sinv_gemm_force_config(cfg), "force_config")
+    try:
+        launch()
+        one_ms = elapsed(launch)
+        n = max(reps, min(64, int(0.2 / max(one_ms, 1e-4)) + 1))
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=side):
+            for _ in range(n):
+                launch()
     finally:
         nat.check(L.sinv_gemm_force_config(-1), "force_config")
-    return e0.elapsed_time(e1) / reps
+    graph.replay()
+    ms = elapsed(graph.replay) / n
+    del graph
+    return ms
 
 
 def tune_shapes(keys, reps=3, candidates=None, install=True):

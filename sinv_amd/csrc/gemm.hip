@@ -15,7 +15,10 @@
 //  * op(A) and op(B) tiles are written to LDS k-major, so each MFMA operand
 //    read is 16 consecutive complex per quarter-wave;
 //  * the linear workgroup id is remapped so that consecutive tiles (same GEMM,
-//    neighbouring batch entries) land on the same XCD and share its L2.
+//    neighbouring batch entries) land on the same XCD and share its L2;
+//  * smaller configurations (32 x 32 down to 16 x 16, some splitting k among
+//    their waves) serve products too small or too few to fill 256 CUs with
+//    64 x 64 tiles.
 //
 // MFMA 16x16x4 operand layout (lane l of 64): A[i][k] with i = l % 16,
 // k = l / 16; B[k][j] with j = l % 16, k = l / 16.  Accumulator item v holds
@@ -76,6 +79,8 @@ constexpr bool kThreeM = sizeof(R) == 8;
 constexpr int BK = 16;
 constexpr int PAD = 1;
 
+constexpr int kGroupM = 8;  // tile rows per band of the grouped tile order
+
 struct TileMap {
   int z_in, z_out, m0, n0;
 };
@@ -96,8 +101,22 @@ __device__ __forceinline__ TileMap map_tile(int tiles_m, int tiles_n, int inner)
   TileMap m;
   m.z_out = z / inner;
   m.z_in = z - m.z_out * inner;
-  m.m0 = (t / tiles_n) * BMt;
-  m.n0 = (t % tiles_n) * BNt;
+  if (tiles_m >= kGroupM && tiles_n >= kGroupM) {
+    // large tile grids (unbatched products): walk the tiles in bands of
+    // kGroupM tile rows, column by column inside a band, so an XCD's
+    // contiguous range covers a compact block of C and reads few panels of
+    // op(A) and op(B) (32 tiles of 32 x 32 over K = 512: 12 panels, 3 MB,
+    // held in the XCD's 4 MB L2, against 18 panels, 4.5 MB, row by row)
+    const int band = t / (kGroupM * tiles_n);
+    const int first = band * kGroupM;
+    const int rows = min(tiles_m - first, kGroupM);
+    const int r = t - band * (kGroupM * tiles_n);
+    m.m0 = (first + r % rows) * BMt;
+    m.n0 = (r / rows) * BNt;
+  } else {
+    m.m0 = (t / tiles_n) * BMt;
+    m.n0 = (t % tiles_n) * BNt;
+  }
   return m;
 }
 
@@ -108,18 +127,24 @@ __device__ __forceinline__ cplx ld_c(const cplx* p) {
 
 // Tile configuration: WM x WN waves, each owning TM x TN MFMA tiles of 16 x 16;
 // workgroup tile (16 WM TM) x (16 WN TN).  OCC is the occupancy the register
-// budget is sized for (waves per SIMD).
-template <int WM_, int WN_, int TM_, int TN_, int OCC_>
+// budget is sized for (waves per SIMD).  WK > 1 splits K inside the
+// workgroup: WK groups of WM x WN waves share the output tile, each takes its
+// own BK-slice of every KC = WK BK chunk, and the groups' accumulators are
+// summed through LDS, in group order, before the epilogue.  That multiplies
+// the workgroups a small or unbatched product can occupy by the tile-area
+// reduction while each wave still issues a full chunk's MFMAs per barrier.
+template <int WM_, int WN_, int TM_, int TN_, int OCC_, int WK_ = 1>
 struct Cfg {
-  static constexpr int WM = WM_, WN = WN_, TM = TM_, TN = TN_, OCC = OCC_;
+  static constexpr int WM = WM_, WN = WN_, TM = TM_, TN = TN_, OCC = OCC_, WK = WK_;
   static constexpr int BM = 16 * WM * TM, BN = 16 * WN * TN;
-  static constexpr int NT = 64 * WM * WN;
+  static constexpr int NT = 64 * WM * WN * WK;
+  static constexpr int KC = BK * WK;  // k per LDS chunk
   // load groups: VA (VB) elements consecutive in memory, GA (GB) groups per thread
-  static constexpr int VA = BM * BK / NT >= 4 ? 4 : BM * BK / NT;
-  static constexpr int VB = BN * BK / NT >= 4 ? 4 : BN * BK / NT;
-  static constexpr int GA = BM * BK / VA / NT;
-  static constexpr int GB = BN * BK / VB / NT;
-  static_assert(VA >= 1 && VB >= 1 && GA * VA * NT == BM * BK && GB * VB * NT == BN * BK,
+  static constexpr int VA = BM * KC / NT >= 4 ? 4 : BM * KC / NT;
+  static constexpr int VB = BN * KC / NT >= 4 ? 4 : BN * KC / NT;
+  static constexpr int GA = BM * KC / VA / NT;
+  static constexpr int GB = BN * KC / VB / NT;
+  static_assert(VA >= 1 && VB >= 1 && GA * VA * NT == BM * KC && GB * VB * NT == BN * KC,
                 "tile must split evenly into load groups");
 };
 
@@ -127,15 +152,24 @@ template <typename R, typename C>
 __global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::OCC)))
 void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
   constexpr int BMt = C::BM, BNt = C::BN, NT = C::NT, TM = C::TM, TN = C::TN;
-  constexpr int GA = C::GA, GB = C::GB, VA = C::VA, VB = C::VB;
-  __shared__ CR<R> As2[2][BK][BMt + PAD];
-  __shared__ CR<R> Bs2[2][BK][BNt + PAD];
+  constexpr int GA = C::GA, GB = C::GB, VA = C::VA, VB = C::VB, KC = C::KC, WK = C::WK;
+  // one allocation: the two chunk buffers of A and of B, reused after the K
+  // loop for the split-K partial sums (WK > 1)
+  constexpr int kAElems = 2 * KC * (BMt + PAD), kBElems = 2 * KC * (BNt + PAD);
+  constexpr int kRedVals = (WK - 1) * C::WM * C::WN * 64 * 3 * TM * TN * 4;
+  static_assert(kRedVals * sizeof(R) <= (kAElems + kBElems) * sizeof(CR<R>),
+                "split-K partial sums must fit in the chunk buffers");
+  __shared__ CR<R> lds_raw[kAElems + kBElems];
+  auto& As2 = *reinterpret_cast<CR<R>(*)[2][KC][BMt + PAD]>(lds_raw);
+  auto& Bs2 = *reinterpret_cast<CR<R>(*)[2][KC][BNt + PAD]>(lds_raw + kAElems);
 
   const TileMap tm = map_tile<BMt, BNt>(tiles_m, tiles_n, g.inner);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave / C::WN, wn = wave % C::WN;
+  const int wk = wave / (C::WM * C::WN);         // k-slice group (0 when WK == 1)
+  const int wmn = wave - wk * (C::WM * C::WN);
+  const int wm = wmn / C::WN, wn = wmn % C::WN;
 
   const cplx* A = g.A + tm.z_in * g.sA_in + tm.z_out * g.sA_out;
   const cplx* B = g.B + tm.z_in * g.sB_in + tm.z_out * g.sB_out;
@@ -159,8 +193,8 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
   for (int j = 0; j < EA; ++j) {
     if (a_kcontig) {
       const int L = tid + (j / VA) * NT;
-      am[j] = L / (BK / VA);
-      ak[j] = (L % (BK / VA)) * VA + j % VA;
+      am[j] = L / (KC / VA);
+      ak[j] = (L % (KC / VA)) * VA + j % VA;
     } else {
       const int E = tid + j * NT;
       ak[j] = E / BMt;
@@ -176,8 +210,8 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
       bn[j] = E % BNt;
     } else {
       const int L = tid + (j / VB) * NT;
-      bn[j] = L / (BK / VB);
-      bk[j] = (L % (BK / VB)) * VB + j % VB;
+      bn[j] = L / (KC / VB);
+      bk[j] = (L % (KC / VB)) * VB + j % VB;
     }
   }
 
@@ -206,8 +240,8 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
   // always exists, and zeroed when staged into LDS.  All loads of a chunk are
   // then in flight together instead of each behind its own branch.
   auto load_tiles = [&](Regs& X, int k0) {
-    X.fullA = fullM && k0 + BK <= g.K;
-    X.fullB = fullN && k0 + BK <= g.K;
+    X.fullA = fullM && k0 + KC <= g.K;
+    X.fullB = fullN && k0 + KC <= g.K;
     if (X.fullA) {
 #pragma unroll
       for (int j = 0; j < EA; ++j) X.ra[j] = ld_c(pa[j] + k0 * a_kstep);
@@ -287,7 +321,7 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
     auto& Bs = Bs2[buf];
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      const int kr = kk + (lane >> 4);
+      const int kr = wk * BK + kk + (lane >> 4);
       CR<R> a[TM], b[TN];
 #pragma unroll
       for (int q = 0; q < TM; ++q) a[q] = As[kr][wm * TM * 16 + q * 16 + (lane & 15)];
@@ -333,16 +367,55 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
     }
   };
 
-  const int ktiles = (g.K + BK - 1) / BK;
+  const int ktiles = (g.K + KC - 1) / KC;
   load_tiles(R0, 0);
   store_tiles(R0, 0);
   __syncthreads();
   for (int kt = 0; kt < ktiles; ++kt) {
     const bool more = kt + 1 < ktiles;
-    if (more) load_tiles(R0, (kt + 1) * BK);
+    if (more) load_tiles(R0, (kt + 1) * KC);
     compute(kt & 1);
     if (more) store_tiles(R0, (kt + 1) & 1);
     __syncthreads();
+  }
+
+  if constexpr (WK > 1) {
+    // split-K: groups 1 .. WK-1 park their accumulators in LDS (the last
+    // barrier of the K loop has retired every chunk read); group 0 adds them
+    // in group order, so the sum is the same on every run, and writes C.
+    constexpr int NV = 3 * TM * TN * 4;  // accumulator values per lane
+    R* red = reinterpret_cast<R*>(lds_raw);
+    if (wk > 0) {
+      R* dst = red + ((wk - 1) * (C::WM * C::WN) + wmn) * NV * 64 + lane;
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int o = ((i * TN + j) * 4 + v) * 3;
+            dst[(o + 0) * 64] = t1[i][j][v];
+            dst[(o + 1) * 64] = t2[i][j][v];
+            dst[(o + 2) * 64] = t3[i][j][v];
+          }
+    }
+    __syncthreads();
+    if (wk > 0) return;
+#pragma unroll
+    for (int s = 1; s < WK; ++s) {
+      const R* src = red + ((s - 1) * (C::WM * C::WN) + wmn) * NV * 64 + lane;
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int o = ((i * TN + j) * 4 + v) * 3;
+            t1[i][j][v] += src[(o + 0) * 64];
+            t2[i][j][v] += src[(o + 1) * 64];
+            t3[i][j][v] += src[(o + 2) * 64];
+          }
+    }
   }
 
   // epilogue
@@ -440,11 +513,19 @@ using Cfg128x64 = Cfg<4, 2, 2, 2, 2>; // 3: 128 x 64, 8 waves
 using Cfg64x128 = Cfg<2, 4, 2, 2, 2>; // 4: 64 x 128, 8 waves
 using Cfg64x32 = Cfg<2, 2, 2, 1, 3>;  // 5: 64 x 32, 4 waves of 32 x 16 (3 workgroups per CU)
 using Cfg32x64 = Cfg<2, 2, 1, 2, 3>;  // 6: 32 x 64, 4 waves of 16 x 32
-constexpr int kNumCfg = 7;
-constexpr int kCfgBM[kNumCfg] = {Cfg64::BM,     Cfg32w4::BM,   Cfg32w1::BM, Cfg128x64::BM,
-                                 Cfg64x128::BM, Cfg64x32::BM, Cfg32x64::BM};
-constexpr int kCfgBN[kNumCfg] = {Cfg64::BN,     Cfg32w4::BN,   Cfg32w1::BN, Cfg128x64::BN,
-                                 Cfg64x128::BN, Cfg64x32::BN, Cfg32x64::BN};
+// split-K inside the workgroup, for products too small or too few to fill the
+// chip with output tiles alone (unbatched 64^3 .. 512^3)
+using Cfg32k4 = Cfg<1, 1, 2, 2, 1, 4>;  // 7: 32 x 32, 4 k-slices of 32 x 32 waves
+using Cfg16k4 = Cfg<1, 1, 1, 1, 2, 4>;  // 8: 16 x 16, 4 k-slices
+using Cfg16k8 = Cfg<1, 1, 1, 1, 1, 8>;  // 9: 16 x 16, 8 k-slices
+constexpr int kNumCfg = 10;
+constexpr int kCfgBM[kNumCfg] = {Cfg64::BM,     Cfg32w4::BM,  Cfg32w1::BM,  Cfg128x64::BM,
+                                 Cfg64x128::BM, Cfg64x32::BM, Cfg32x64::BM, Cfg32k4::BM,
+                                 Cfg16k4::BM,   Cfg16k8::BM};
+constexpr int kCfgBN[kNumCfg] = {Cfg64::BN,     Cfg32w4::BN,  Cfg32w1::BN,  Cfg128x64::BN,
+                                 Cfg64x128::BN, Cfg64x32::BN, Cfg32x64::BN, Cfg32k4::BN,
+                                 Cfg16k4::BN,   Cfg16k8::BN};
+constexpr int kCfgWK[kNumCfg] = {1, 1, 1, 1, 1, 1, 1, Cfg32k4::WK, Cfg16k4::WK, Cfg16k8::WK};
 
 template <typename R, typename C>
 void launch_cfg(const GemmArgs& g, long long batch, hipStream_t s) {
@@ -463,6 +544,9 @@ void launch_by_id(int id, const GemmArgs& g, long long batch, hipStream_t s) {
     case 4: launch_cfg<R, Cfg64x128>(g, batch, s); break;
     case 5: launch_cfg<R, Cfg64x32>(g, batch, s); break;
     case 6: launch_cfg<R, Cfg32x64>(g, batch, s); break;
+    case 7: launch_cfg<R, Cfg32k4>(g, batch, s); break;
+    case 8: launch_cfg<R, Cfg16k4>(g, batch, s); break;
+    case 9: launch_cfg<R, Cfg16k8>(g, batch, s); break;
     default: launch_cfg<R, Cfg64>(g, batch, s); break;
   }
 }
@@ -492,10 +576,15 @@ int tune_find(const GemmArgs& g, long long batch, int prec) {
 
 // Untuned shapes: the 64 x 64 tile, unless its grid leaves the chip under half
 // full (under 2 workgroups per CU), where 32 x 32 tiles of 4 waves give 4x the
-// workgroups.
+// workgroups; if even those leave CUs idle (an unbatched product of 256^3 or
+// less), 16 x 16 tiles with k split four ways inside the workgroup (measured
+// 64^3 4.2 us, 128^3 5.2 us, 256^3 7.5 us per launch against 5.8, 8.8 and
+// 15.0 us with the 32 x 32 tile).
 int default_cfg(const GemmArgs& g, long long batch) {
   const long long t64 = (long long)((g.M + 63) / 64) * ((g.N + 63) / 64) * batch;
-  return t64 < 512 ? 1 : 0;
+  if (t64 >= 512) return 0;
+  const long long t32 = (long long)((g.M + 31) / 32) * ((g.N + 31) / 32) * batch;
+  return (t32 < 256 && g.K >= 48) ? 8 : 1;
 }
 
 int g_force_cfg = -1;  // >= 0: every launch uses this config (tuning runs)
@@ -711,6 +800,15 @@ extern "C" int sinv_gemm_config_tile(int cfg, int* bm, int* bn) {
   *bm = kCfgBM[cfg];
   *bn = kCfgBN[cfg];
   return SINV_OK;
+}
+
+extern "C" int sinv_gemm_config_ksplit(int cfg) {
+  using namespace sinv;
+  if (cfg < 0 || cfg >= kNumCfg) {
+    set_error("sinv_gemm_config_ksplit: bad config id");
+    return -SINV_E_INVAL;  // negative: never a k-split count
+  }
+  return kCfgWK[cfg];
 }
 
 extern "C" int sinv_gemm_force_config(int cfg) {

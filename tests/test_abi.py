@@ -88,9 +88,14 @@ def test_gemm_tune_table_host_side(built_lib):
     assert built_lib.sinv_gemm_tune_set(*key, n) != 0
     built_lib.sinv_gemm_tune_clear()
     assert built_lib.sinv_gemm_select(*key) == default
-    # untuned: small grids get the 32 x 32 tile, large ones the 64 x 64 tile
+    # untuned: large grids get the 64 x 64 tile, small ones the 32 x 32 tile,
+    # and products too small to give every CU a 32 x 32 tile split k four ways
     assert tiles[built_lib.sinv_gemm_select(512, 512, 512, 1, 0, 0, 0, 0)] == (32, 32)
     assert tiles[built_lib.sinv_gemm_select(384, 384, 384, 256, 0, 0, 0, 0)] == (64, 64)
+    small = built_lib.sinv_gemm_select(128, 128, 128, 1, 0, 0, 0, 0)
+    assert tiles[small] == (16, 16) and built_lib.sinv_gemm_config_ksplit(small) == 4
+    assert built_lib.sinv_gemm_config_ksplit(0) == 1
+    assert built_lib.sinv_gemm_config_ksplit(n) < 0
 
 
 def test_autotune_table_save_load(tmp_path, built_lib):

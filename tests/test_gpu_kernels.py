@@ -255,9 +255,11 @@ def test_zinv_nan_input_propagates(built_lib):
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (65, 33, 17), (128, 96, 200), (130, 140, 40)])
 @pytest.mark.parametrize("opA,opB", [("N", "N"), ("T", "N"), ("N", "C"), ("C", "T")])
 def test_zgemm_every_tile_config(built_lib, M, N, K, opA, opB):
-    """Every tile configuration against numpy, and all of them bit-identical to
-    each other: they differ only in which workgroup computes which outputs, the
-    summation over k is the same chunked MFMA sequence in each."""
+    """Every tile configuration against numpy, and configs with the same k-split
+    bit-identical to each other: they differ only in which workgroup computes
+    which outputs, the summation over k is the same chunked MFMA sequence in
+    each.  Configs with different k-splits sum the same products in another
+    order and agree to rounding."""
     from sinv_amd import autotune, core
     import torch
     rng = np.random.default_rng(M + 7 * N + 13 * K)
@@ -266,7 +268,7 @@ def test_zgemm_every_tile_config(built_lib, M, N, K, opA, opB):
     B = _rand(rng, batch, *((K, N) if opB == "N" else (N, K)))
     C0 = _rand(rng, batch, M, N)
     ref = 0.5 * (_op(A, opA) @ _op(B, opB)) - C0
-    outs = []
+    outs = {}
     for cfg, tile in autotune.configs():
         print(f"zgemm cfg={cfg} tile={tile} M={M} N={N} K={K} {opA}{opB}", flush=True)
         built_lib.sinv_gemm_force_config(cfg)
@@ -278,16 +280,25 @@ def test_zgemm_every_tile_config(built_lib, M, N, K, opA, opB):
                 err = np.abs(got - ref).max() / np.abs(ref).max()
                 assert err < (1e-13 if prec == "fp64" else 1e-5), (cfg, prec, err)
                 if prec == "fp64":
-                    outs.append(C)
+                    outs[cfg] = C
         finally:
             built_lib.sinv_gemm_force_config(-1)
-    for cfg, C in enumerate(outs[1:], 1):
-        assert torch.equal(C, outs[0]), cfg
+    first = {}
+    for cfg, C in outs.items():
+        ks = built_lib.sinv_gemm_config_ksplit(cfg)
+        assert ks >= 1
+        if ks in first:
+            assert torch.equal(C, outs[first[ks]]), (cfg, first[ks])
+        else:
+            first[ks] = cfg
+    assert len(first) >= 2   # sequential and split-k configs were both exercised
 
 
 def test_autotune_installs_and_keeps_results(built_lib):
     """Tuning a small batched RGF installs one config per GEMM shape (the one
-    that timed fastest) and leaves the sweep's results bit-identical."""
+    that timed fastest).  The sweep's results change at most by rounding (a
+    tuned config may split k differently from the default one) and are
+    bit-identical from run to run under the tuned table."""
     import torch
     from sinv_amd import autotune, core
     from sinv_amd.utils import configs
@@ -307,7 +318,12 @@ def test_autotune_installs_and_keeps_results(built_lib):
         plan.run()
         torch.cuda.synchronize()
         plan.check()
-        for a, b in zip(plan.outputs(), before):
+        tuned = tuple(x.clone() for x in plan.outputs())
+        for a, b in zip(tuned, before):
+            assert float((a - b).abs().max()) <= 1e-12 * float(b.abs().max())
+        plan.run()
+        torch.cuda.synchronize()
+        for a, b in zip(plan.outputs(), tuned):
             assert torch.equal(a, b)
     finally:
         autotune.clear()
