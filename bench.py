@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--chunk", type=int, default=64, help="energies per adjoint launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the batch=1 FP64 sample")
+    ap.add_argument("--no-pcie", action="store_true",
+                    help="skip the PCIe-inclusive figure (pinned copy rates, drop-in call)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the GEMM tile autotuner (untimed, before the timed steps)")
     ap.add_argument("--save-tune", default=None, help="write the tuned GEMM table (JSON) here")
@@ -337,6 +339,47 @@ def main():
     if err_mixed <= ERR_GATE and mixed_sps > fp64_sps:
         head = "mixed"
     ms = results[head]["ms"]
+
+    # ------------------------------------------------------------ PCIe-inclusive rate (never `value`)
+    pcie = None
+    if not args.no_pcie:
+        n = 1 << 28  # 2 GiB of float64 between pinned host memory and HBM
+        hbuf = torch.empty(n, dtype=torch.float64, pin_memory=True)
+        dbuf = torch.empty(n, dtype=torch.float64, device=dev)
+
+        def copy_rate(fn, reps=3):
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            torch.cuda.synchronize()
+            return reps * 8 * n / (time.perf_counter() - t0) / 1e9
+
+        h2d = copy_rate(lambda: dbuf.copy_(hbuf, non_blocking=True))
+        d2h = copy_rate(lambda: hbuf.copy_(dbuf, non_blocking=True))
+        del hbuf, dbuf
+        blocks = lambda g: (3 * g["N"] - 2) * g["b"] ** 2 * 16  # complex128 band of one instance
+        bytes_in = sum(blocks(g) for g in groups)  # H (or A) once; energies are a few bytes
+        bytes_out = sum((len(g["shift"]) if g["shift"] is not None else 1) * blocks(g)
+                        for g in groups)
+        t_io = bytes_in / (h2d * 1e9) + bytes_out / (d2h * 1e9)
+        pcie = dict(solves_per_s=round(solves / (ms / 1e3 + t_io), 3), h2d_GBps=round(h2d, 1),
+                    d2h_GBps=round(d2h, 1), bytes_in=bytes_in, bytes_out=bytes_out,
+                    method="headline step + the input band to the device + every selected block "
+                           "back to pinned host memory, at the measured copy rates, not overlapped")
+        if len(groups) == 1 and groups[0]["shift"] is None:
+            # the drop-in itself: host container in, host container out (rgf.py's call surface)
+            from sinv_amd.algorithms import rgf as rgf_dropin
+            from sinv_amd.bsparse import BlockTridiag
+            A_host = BlockTridiag(*groups[0]["H"])
+            rgf_dropin.rgf(A_host)
+            t0 = time.perf_counter()
+            rgf_dropin.rgf(A_host)
+            pcie["dropin_call_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+            del A_host
+        log(f"[bench] PCIe-inclusive: {pcie}")
+
     kind = "gemm_f64" if head == "fp64" else "gemm_f32"
     kms, kcnt, kfl = results[head]["prof"][kind]
     kname = f"gemm_kernel<{'double' if head == 'fp64' else 'float'}>"
@@ -380,6 +423,7 @@ def main():
         "speedup_vs_plain_fp64": (round(solves / (ms / 1e3) / plain["solves_per_s"], 2) if plain else None),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "pcie_inclusive": pcie,
         "gemm_autotune": tuned or None,
         "kernel_time": {k: dict(ms=round(v[0], 3), launches=v[1], tflops=round(v[2] / (v[0] / 1e3) / 1e12, 3)
                                 if v[0] > 0 else 0.0) for k, v in results[head]["prof"].items() if v[1]},
