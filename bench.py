@@ -302,18 +302,26 @@ def main():
                      sample=f"{nsamp} energies per block size, batch=1 launch sequences")
 
     # ------------------------------------------------------------ CPU baseline (oracle, sampled)
+    # numpy restatement of rgf.py (the reference's own inv / @ calls), on the
+    # host's cores: median of 5 solves after a warm-up per block size, all
+    # BLAS threads; plus one single-thread solve per block size on at most 16
+    # blocks, scaled to the block count by the flop count (linear in N)
     cpu = None
     if not args.no_cpu:
         from oracle import rgf_ref
         cores = len(os.sched_getaffinity(0))
+        blas = {}
+        limit1 = None
         try:
-            from threadpoolctl import threadpool_info
-            blas = [i for i in threadpool_info() if i.get("user_api") == "blas"]
-            if blas:
-                cores = int(blas[0]["num_threads"])
+            from threadpoolctl import threadpool_info, threadpool_limits
+            info = [i for i in threadpool_info() if i.get("user_api") == "blas"]
+            if info:
+                cores = int(info[0]["num_threads"])
+                blas = {k: info[0].get(k) for k in ("internal_api", "version", "threading_layer")}
+            limit1 = threadpool_limits
         except Exception:
             pass
-        t_cpu = 0.0
+        t_cpu = t_cpu1 = 0.0
         parts = []
         for g in groups:
             nE = len(g["shift"]) if g["shift"] is not None else 1
@@ -321,16 +329,29 @@ def main():
                 d, u, l = configs.shifted(g["H"], g["shift"][0])
             else:
                 d, u, l = g["H"]
-            rgf_ref.rgf_blocks(d[:2], u[:1], l[:1])  # warm BLAS
-            t0 = time.perf_counter()
-            rgf_ref.rgf_blocks(d, u, l)
-            t = time.perf_counter() - t0
-            parts.append(f"b={g['b']}: {t:.2f} s/solve")
+            rgf_ref.rgf_blocks(d, u, l)  # warm-up
+            runs = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                rgf_ref.rgf_blocks(d, u, l)
+                runs.append(time.perf_counter() - t0)
+            t = float(np.median(runs))
             t_cpu += nE * t
-            log(f"[bench] cpu oracle b={g['b']}: {t:.2f} s/solve")
+            msg = f"b={g['b']}: {t:.3f} s/solve (median of 5, {cores} threads)"
+            if limit1 is not None:
+                Ns = min(g["N"], 16)
+                with limit1(limits=1, user_api="blas"):
+                    t0 = time.perf_counter()
+                    rgf_ref.rgf_blocks(d[:Ns], u[:Ns - 1], l[:Ns - 1])
+                    t1 = (time.perf_counter() - t0) * (9 * g["N"] - 8) / (9 * Ns - 8)
+                t_cpu1 += nE * t1
+                msg += f", {t1:.3f} s/solve on 1 thread ({Ns} of {g['N']} blocks, scaled)"
+            parts.append(msg)
+            log(f"[bench] cpu oracle {msg}")
         cpu = dict(value=solves / t_cpu, unit="solves/s", cores=cores, kind="port",
-                   sample="one solve per block size (numpy restatement of rgf.py, multithreaded "
-                          "OpenBLAS), scaled to the step's energy counts; " + ", ".join(parts))
+                   single_thread_value=(solves / t_cpu1 if t_cpu1 > 0 else None), blas=blas or None,
+                   sample="numpy restatement of rgf.py (same inv / @ calls), one instance per "
+                          "block size, scaled to the step's energy counts; " + "; ".join(parts))
 
     # ------------------------------------------------------------ pick the headline
     fp64_sps = solves / (results["fp64"]["ms"] / 1e3)
