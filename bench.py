@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--chunk", type=int, default=64, help="energies per adjoint launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the batch=1 FP64 sample")
+    ap.add_argument("--no-bcr", action="store_true",
+                    help="skip BCR-serial on the single-instance workloads")
     ap.add_argument("--no-pcie", action="store_true",
                     help="skip the PCIe-inclusive figure (pinned copy rates, drop-in call)")
     ap.add_argument("--no-tune", action="store_true",
@@ -401,6 +403,52 @@ def main():
             del A_host
         log(f"[bench] PCIe-inclusive: {pcie}")
 
+    # ------------------------------------------------------------ BCR-serial on the same instance
+    # (single-instance workloads): GPU time, its distance to RGF, and the numpy
+    # restatement of bcr_serial.py in the reference's dense storage where that fits
+    bcr = None
+    if len(groups) == 1 and groups[0]["shift"] is None and not args.no_bcr:
+        g = groups[0]
+        d, u, l = g["dev"]
+        core.bcr_batched(d, u, l)
+        torch.cuda.synchronize()
+        reps = 5
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(reps):
+            Gb = core.bcr_batched(d, u, l, check=False)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        t_gpu = ev0.elapsed_time(ev1) / reps
+        Gr = core.rgf_batched(d, u, l)
+        num = sum(float(torch.linalg.vector_norm(a - r)) ** 2 for a, r in zip(Gb, Gr))
+        den = sum(float(torch.linalg.vector_norm(r)) ** 2 for r in Gr)
+        from oracle import bcr_ref
+        bcr = dict(ms_per_solve=round(t_gpu, 3), solves_per_s=round(1e3 / t_gpu, 3),
+                   rel_err_vs_rgf=float(np.sqrt(num / den)),
+                   executed_flops=8 * g["b"] ** 3 * sum(bcr_ref.bcr_flops(g["N"], g["b"])))
+        del Gb, Gr
+        n = g["N"] * g["b"]
+        npad = (g["N"] + bcr_ref.distance_to_power_of_two(g["N"])) * g["b"]
+        if not args.no_cpu and 4 * npad * npad * 16 <= 2 ** 33:  # four dense n_pad^2 arrays <= 8 GiB
+            from sinv_amd.utils import matu
+            A = matu.convertBlkTridiagToDense(*g["H"])
+            bcr_ref.bcr_serial_dense(A, g["b"])  # warm-up
+            runs = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                bcr_ref.bcr_serial_dense(A, g["b"])
+                runs.append(time.perf_counter() - t0)
+            bcr["cpu_dense_ms_per_solve"] = round(float(np.median(runs)) * 1e3, 3)
+            bcr["cpu_sample"] = (f"numpy restatement of bcr_serial.py on the dense n = {n} matrix "
+                                 f"(padded to {npad}), median of 5, all BLAS threads")
+        else:
+            bcr["cpu_dense_ms_per_solve"] = None
+            bcr["cpu_sample"] = (f"not run: the reference's dense storage needs four {npad}^2 "
+                                 f"complex arrays ({4 * npad * npad * 16 / 1e9:.0f} GB)")
+        log(f"[bench] bcr-serial: {bcr}")
+
     kind = "gemm_f64" if head == "fp64" else "gemm_f32"
     kms, kcnt, kfl = results[head]["prof"][kind]
     kname = f"gemm_kernel<{'double' if head == 'fp64' else 'float'}>"
@@ -445,6 +493,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "pcie_inclusive": pcie,
+        "bcr_serial": bcr,
         "gemm_autotune": tuned or None,
         "kernel_time": {k: dict(ms=round(v[0], 3), launches=v[1], tflops=round(v[2] / (v[0] / 1e3) / 1e12, 3)
                                 if v[0] > 0 else 0.0) for k, v in results[head]["prof"].items() if v[1]},
