@@ -34,6 +34,7 @@ import numpy as np
 METRIC = ("solves/s + TFLOP/s: batched mixed-prec RGF fwd+bwd vs plain FP64 RGF, "
           "err≤1e-8")
 PEAK_TFLOPS = {"gemm_f64": 78.6, "gemm_f32": 157.3}
+HBM_TBPS = 8.0  # MI355X HBM3E
 ERR_GATE = 1e-8
 
 
@@ -466,6 +467,11 @@ def main():
                                          / PEAK_TFLOPS[kind], 4),
                 traffic_source=f"profiles/traffic_{wl['workload']}.txt (rocprofv3 --pmc "
                                "FETCH_SIZE + WRITE_SIZE, calibrated; null if absent)")
+    if roof["traffic"] is not None and kcnt:
+        # HBM side of the same kernel: measured bytes per launch over its launch time
+        hbm = roof["traffic"] / (kms / kcnt / 1e3) / 1e9
+        roof["hbm_GBps"] = round(hbm, 1)
+        roof["hbm_frac"] = round(hbm / (HBM_TBPS * 1e3), 4)
     out = {
         "metric": METRIC,
         "value": round(solves / (ms / 1e3), 3),
