@@ -404,6 +404,22 @@ def main():
             del A_host
         log(f"[bench] PCIe-inclusive: {pcie}")
 
+    # ------------------------------------------------------------ path-level roofline per mode
+    # t_min = F_gemm / P_gemm + F_inv / P_64 over the step's algorithmic work
+    # (reference convention: N inverses and 8(N-1) complex GEMMs of b^3 per
+    # solve); the mixed mode runs its Schur GEMMs on FP32 matrix cores and its
+    # inverses in FP64
+    f_inv = sum((len(g["shift"]) if g["shift"] is not None else 1) * 8.0 * g["b"] ** 3 * g["N"]
+                for g in groups)
+    f_gemm = alg_flops - f_inv
+    path_roof = {}
+    for prec, p_gemm in (("fp64", PEAK_TFLOPS["gemm_f64"]), ("mixed", PEAK_TFLOPS["gemm_f32"])):
+        t_min = (f_gemm / (p_gemm * 1e12) + f_inv / (PEAK_TFLOPS["gemm_f64"] * 1e12)) * 1e3
+        path_roof[prec] = dict(t_min_ms=round(t_min, 3), frac=round(t_min / results[prec]["ms"], 4))
+    path_roof["work"] = dict(gemm_flops=f_gemm, inverse_flops=f_inv,
+                             note="FP64 counts complex products as 8MNK; its 3M GEMM executes 3/4 "
+                                  "of that, so frac can exceed 1")
+
     # ------------------------------------------------------------ BCR-serial on the same instance
     # (single-instance workloads): GPU time, its distance to RGF, and the numpy
     # restatement of bcr_serial.py in the reference's dense storage where that fits
@@ -498,6 +514,7 @@ def main():
         "speedup_vs_plain_fp64": (round(solves / (ms / 1e3) / plain["solves_per_s"], 2) if plain else None),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "path_roofline": path_roof,
         "pcie_inclusive": pcie,
         "bcr_serial": bcr,
         "gemm_autotune": tuned or None,
