@@ -14,6 +14,8 @@ Measured:
   * batched FP64 RGF (complex128 GEMMs on FP64 MFMA, FP64 inverses)
   * batched mixed RGF (complex64 GEMMs on FP32 MFMA, FP64 inverses) and its
     error against the FP64 result over every instance
+  * batched int8x RGF (complex128 GEMMs emulated from 7 int8 digit planes per
+    operand on the int8 matrix cores, FP64 inverses) and its error likewise
   * plain FP64 RGF, one energy per launch sequence (batch = 1), on a sample
   * the CPU baseline: the oracle's numpy restatement of rgf.py on the host
 `value` is the solves/s of the fastest batched mode whose max relative error
@@ -33,7 +35,8 @@ import numpy as np
 
 METRIC = ("solves/s + TFLOP/s: batched mixed-prec RGF fwd+bwd vs plain FP64 RGF, "
           "err≤1e-8")
-PEAK_TFLOPS = {"gemm_f64": 78.6, "gemm_f32": 157.3}
+PEAK_TFLOPS = {"gemm_f64": 78.6, "gemm_f32": 157.3, "gemm_i8x": 5000.0}  # int8: TOP/s
+I8X_OPS_PER_FLOP = 21.0  # int8 MFMA ops per algorithmic flop (3 x 28 digit pairs x 2MNK / 8MNK)
 HBM_TBPS = 8.0  # MI355X HBM3E
 ERR_GATE = 1e-8
 
@@ -81,6 +84,20 @@ def measured_traffic(workload, kernel):
         if len(f) == 5 and f[0] == kernel:
             return float(f[4]) * 1e6
     return None
+
+
+def _kind_summary(res, kind):
+    """Launch-profiler totals of one kernel kind in a profiled step."""
+    kms, kcnt, kfl = res["prof"].get(kind, (0.0, 0, 0.0))
+    if not kcnt:
+        return None
+    tf = kfl / (kms / 1e3) / 1e12 if kms > 0 else 0.0
+    d = dict(ms=round(kms, 3), launches=int(kcnt), avg_launch_us=round(kms / kcnt * 1e3, 3),
+             algorithmic_tflops=round(tf, 3))
+    if kind == "gemm_i8x":
+        d["int8_tops_executed"] = round(tf * I8X_OPS_PER_FLOP, 1)
+        d["int8_peak_frac"] = round(tf * I8X_OPS_PER_FLOP / PEAK_TFLOPS["gemm_i8x"], 4)
+    return d
 
 
 def rel_err_per_instance(G, R):
@@ -207,10 +224,12 @@ def main():
 
     results = {}
     tuned = {}
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64", "mixed", "int8x"):
         log(f"[bench] {wl['workload']}: batched {prec}: allocating plans")
         plans = make_plans(prec)
-        if args.load_tune:
+        if prec == "int8x":
+            pass  # one kernel shape for every product: nothing to tune
+        elif args.load_tune:
             from sinv_amd import autotune
             autotune.load(args.load_tune)
         elif not args.no_tune:
@@ -240,8 +259,9 @@ def main():
         torch.cuda.synchronize()
 
     # ------------------------------------------------------------ error of mixed vs FP64, all instances
-    log("[bench] error of mixed vs fp64 over every instance")
+    log("[bench] error of mixed and int8x vs fp64 over every instance")
     err_mixed = 0.0
+    err_i8x = 0.0
     chunk = 16
     for gi, g in enumerate(groups):
         nE = len(g["shift"]) if g["shift"] is not None else 1
@@ -249,15 +269,16 @@ def main():
             sl = slice(s0, min(nE, s0 + chunk))
             sh = None if g["shift"] is None else g["shift"][sl]
             outs = []
-            for prec in ("fp64", "mixed"):
+            for prec in ("fp64", "mixed", "int8x"):
                 p = core.RgfPlan(*g["dev"], shift=sh, dscale=g["dscale"], oscale=g["oscale"],
                                  precision=prec, batch=None)
                 p.run()
                 outs.append(p.outputs())
             err_mixed = max(err_mixed, rel_err_per_instance(outs[1], outs[0]))
+            err_i8x = max(err_i8x, rel_err_per_instance(outs[2], outs[0]))
             del outs, p
     torch.cuda.synchronize()
-    log(f"[bench] mixed max rel err vs fp64: {err_mixed:.3e}")
+    log(f"[bench] mixed max rel err vs fp64: {err_mixed:.3e}; int8x: {err_i8x:.3e}")
 
     # ------------------------------------------------------------ FP64 vs the CPU oracle (sample)
     err_fp64 = None
@@ -359,9 +380,12 @@ def main():
     # ------------------------------------------------------------ pick the headline
     fp64_sps = solves / (results["fp64"]["ms"] / 1e3)
     mixed_sps = solves / (results["mixed"]["ms"] / 1e3)
+    i8x_sps = solves / (results["int8x"]["ms"] / 1e3)
     head = "fp64"
     if err_mixed <= ERR_GATE and mixed_sps > fp64_sps:
         head = "mixed"
+    if err_i8x <= ERR_GATE and i8x_sps > max(fp64_sps, mixed_sps if head == "mixed" else 0.0):
+        head = "int8x"
     ms = results[head]["ms"]
 
     # ------------------------------------------------------------ PCIe-inclusive rate (never `value`)
@@ -466,11 +490,16 @@ def main():
                                  f"complex arrays ({4 * npad * npad * 16 / 1e9:.0f} GB)")
         log(f"[bench] bcr-serial: {bcr}")
 
-    kind = "gemm_f64" if head == "fp64" else "gemm_f32"
+    kind = {"fp64": "gemm_f64", "mixed": "gemm_f32", "int8x": "gemm_i8x"}[head]
     kms, kcnt, kfl = results[head]["prof"][kind]
-    kname = f"gemm_kernel<{'double' if head == 'fp64' else 'float'}>"
+    kname = {"fp64": "gemm_kernel<double>", "mixed": "gemm_kernel<float>",
+             "int8x": "gemm_i8x_kernel<7>"}[head]
     achieved = kfl / (kms / 1e3) / 1e12 if kms > 0 else 0.0
-    roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_TFLOPS[kind], unit="TFLOP/s",
+    if head == "int8x":
+        # executed int8 operations: 3 real products x 28 digit pairs x 2 MNK = 21 x 8 MNK
+        achieved *= I8X_OPS_PER_FLOP
+    roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_TFLOPS[kind],
+                unit="TOP/s" if head == "int8x" else "TFLOP/s",
                 frac=round(achieved / PEAK_TFLOPS[kind], 4),
                 traffic=measured_traffic(wl["workload"], kname),
                 kernel=kname, launches=kcnt, avg_launch_us=round(kms / max(kcnt, 1) * 1e3, 3),
@@ -499,7 +528,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "complex128" if head == "fp64" else "complex64-gemm/complex128-inverse",
+        "dtype": {"fp64": "complex128", "mixed": "complex64-gemm/complex128-inverse",
+                  "int8x": "int8-digit-gemm/complex128"}[head],
         "data": "synthetic",
         "config": wl,
         "tflops": round(alg_flops / (ms / 1e3) / 1e12, 3),
@@ -509,6 +539,10 @@ def main():
                                  err_vs_cpu_oracle=err_fp64),
             "batched_mixed": dict(solves_per_s=round(mixed_sps, 3), ms_per_step=round(results["mixed"]["ms"], 3),
                                   max_rel_err_vs_fp64=err_mixed, passes_err_gate=err_mixed <= ERR_GATE),
+            "batched_int8x": dict(solves_per_s=round(i8x_sps, 3), ms_per_step=round(results["int8x"]["ms"], 3),
+                                  max_rel_err_vs_fp64=err_i8x, passes_err_gate=err_i8x <= ERR_GATE,
+                                  digit_planes=7,
+                                  gemm_i8x=_kind_summary(results["int8x"], "gemm_i8x")),
             "plain_fp64_batch1": plain,
         },
         "speedup_vs_plain_fp64": (round(solves / (ms / 1e3) / plain["solves_per_s"], 2) if plain else None),

@@ -40,6 +40,9 @@ extern "C" {
 
 #define SINV_PREC_FP64 0  /* complex128 GEMMs on FP64 MFMA, FP64 inverses   */
 #define SINV_PREC_MIXED 1 /* complex64 GEMMs on FP32 MFMA, FP64 inverses    */
+#define SINV_PREC_INT8X 2 /* complex128 GEMMs from 7 int8 digit planes per   */
+                          /* operand on the int8 matrix cores (exact int32   */
+                          /* sums, ~2^-48 relative), FP64 inverses; RGF only */
 
 #define SINV_OP_N 0
 #define SINV_OP_T 1
@@ -292,6 +295,29 @@ int sinv_zgemm_batched(int M, int N, int K, int opA, int opB, sinv_c128 alpha,
                        sinv_c128 beta, const void* C0, int ldc0, long long strideC0,
                        void* C, int ldc, long long strideC,
                        const void* shift, int batch, int precision, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Complex128 GEMM from int8 digit planes (no counterpart in the reference,
+ * whose `@` is FP64 BLAS; it stands in for the same products, e.g.
+ * rgf.py:47,52,58,60).  Same contract as sinv_zgemm_batched.  Each row of
+ * op(A) and column of op(B) is scaled by a power of two and split into
+ * `slices` (5..7) signed 7-bit digits per real part (re, im, re + im); digit
+ * products are summed exactly in int32 on the int8 matrix cores and pairs
+ * with digit index sum >= slices are dropped, so one real product errs by at
+ * most about 2^(1 - 7 slices) K max|row of op(A)| max|column of op(B)|.
+ * K <= 16384.  The workspace (size from sinv_zgemm_int8x_workspace_bytes with
+ * the same strides; stride 0 = one operand shared by the batch, sliced once)
+ * receives the digit planes; nothing is allocated.
+ * ------------------------------------------------------------------------ */
+size_t sinv_zgemm_int8x_workspace_bytes(int M, int N, int K, long long strideA,
+                                        long long strideB, int batch, int slices);
+int sinv_zgemm_int8x_batched(int M, int N, int K, int opA, int opB, sinv_c128 alpha,
+                             const void* A, int lda, long long strideA,
+                             const void* B, int ldb, long long strideB,
+                             sinv_c128 beta, const void* C0, int ldc0, long long strideC0,
+                             void* C, int ldc, long long strideC,
+                             const void* shift, int batch, int slices,
+                             void* work, size_t work_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
  * Dense <-> block-tridiagonal conversion on the device.

@@ -16,7 +16,7 @@ SINV_E_INVAL = 2
 SINV_E_HIP = 3
 SINV_INFO_OK = (1 << 64) - 1
 
-PREC = {"fp64": 0, "mixed": 1}
+PREC = {"fp64": 0, "mixed": 1, "int8x": 2}
 OPS = {"N": 0, "T": 1, "C": 2}
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsinv_hip.so")
@@ -133,6 +133,12 @@ def lib():
     L.sinv_zgemm_batched.restype = _i
     L.sinv_zgemm_batched.argtypes = [_i, _i, _i, _i, _i, C128, _vp, _i, _ll, _vp, _i, _ll,
                                      C128, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _i, _vp]
+    L.sinv_zgemm_int8x_workspace_bytes.restype = ctypes.c_size_t
+    L.sinv_zgemm_int8x_workspace_bytes.argtypes = [_i, _i, _i, _ll, _ll, _i, _i]
+    L.sinv_zgemm_int8x_batched.restype = _i
+    L.sinv_zgemm_int8x_batched.argtypes = [_i, _i, _i, _i, _i, C128, _vp, _i, _ll, _vp, _i, _ll,
+                                           C128, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _i, _vp,
+                                           ctypes.c_size_t, _vp]
     L.sinv_prof_begin.restype = _i
     L.sinv_prof_begin.argtypes = [_i]
     L.sinv_prof_end.restype = _i
@@ -215,7 +221,7 @@ def check_info(info_tensor, what="inverse"):
         raise err
 
 
-PROF_KINDS = ("gemm_f64", "gemm_f32", "inv_leaf", "other")
+PROF_KINDS = ("gemm_f64", "gemm_f32", "inv_leaf", "other", "gemm_i8x")
 
 # leaf inverse kernels (include/sinv.h SINV_LEAF_*)
 LEAF_VARIANTS = {"auto": 1, "lds": 0, "rows8": 2, "panel8": 3, "panel16": 4, "panel16f": 5,

@@ -642,8 +642,11 @@ def zinv_batched(A, check=True):
 
 
 def zgemm_batched(A, B, *, opA="N", opB="N", alpha=1.0, beta=0.0, C0=None, shift=None,
-                  precision="fp64"):
-    """C = alpha op(A) op(B) + beta C0 (+ shift I) over a batch (leading dim) of blocks."""
+                  precision="fp64", slices=7):
+    """C = alpha op(A) op(B) + beta C0 (+ shift I) over a batch (leading dim) of blocks.
+
+    precision "int8x" forms the complex128 product from `slices` int8 digit
+    planes per operand on the int8 matrix cores (sinv_zgemm_int8x_batched)."""
     A = to_dev(A)
     B = to_dev(B)
     if A.dim() == 2:
@@ -669,6 +672,13 @@ def zgemm_batched(A, B, *, opA="N", opB="N", alpha=1.0, beta=0.0, C0=None, shift
         sC0 = 0
     sh = to_dev(shift).reshape(-1) if shift is not None else None
     L = nat.lib()
+    if precision == "int8x":
+        ws = _workspace(L.sinv_zgemm_int8x_workspace_bytes(M, N, K, sA, sB, batch, slices))
+        nat.check(L.sinv_zgemm_int8x_batched(
+            M, N, K, nat.OPS[opA], nat.OPS[opB], nat.C128.of(alpha), _p(A), ac, sA, _p(B), bc,
+            sB, nat.C128.of(beta), _p(C0), N, sC0, _p(C), N, M * N, _p(sh), batch, slices,
+            _p(ws), ws.numel(), _stream()), "zgemm_int8x")
+        return C
     nat.check(L.sinv_zgemm_batched(
         M, N, K, nat.OPS[opA], nat.OPS[opB], nat.C128.of(alpha), _p(A), ac, sA, _p(B), bc, sB,
         nat.C128.of(beta), _p(C0), N, sC0, _p(C), N, M * N, _p(sh), batch, nat.PREC[precision],

@@ -1,0 +1,631 @@
+// FP64-accurate complex GEMM on the int8 matrix cores (sliced-integer
+// emulation in the manner of Ozaki, Ogita and Oishi's error-free splitting).
+//
+// C[z] = alpha * op(A[z]) op(B[z]) + beta * C0[z] + shift[z] * I, complex128
+// in and out, the same contract as launch_gemm.
+//
+// Method.  Each row r of op(A) (each column of op(B)) and each real part
+// p in {re, im, re + im} gets a power-of-two scale 2^e with max |x| < 2^e over
+// the row, and the scaled values x / 2^e in (-1, 1) are split into S signed
+// 7-bit digits by exact double arithmetic:
+//     y <- 128 y;  q_t = trunc(y);  y <- y - q_t     (t = 0 .. S-1)
+// so x = 2^e (sum_t q_t 2^(-7(t+1)) + rem), |rem| < 2^(-7S), |q_t| <= 127.
+// The digit products q_t(A) q_u(B) summed over k are exact in int32 on
+// v_mfma_i32_16x16x64_i8; pairs with the same t + u share one accumulator, and
+// pairs with t + u > S - 1 are dropped.  After the k loop each accumulator is
+// converted to double and scaled by 2^(-7(t+u+2)) 2^(eA_r + eB_c).  Per term
+// the dropped pairs (S - 1 of size 2^(-7S) at most, then geometrically
+// smaller) and the two remainders beyond S digits give an error below
+// 4 (S + 1) 2^(-7S) max_k|a_rk| max_k|b_kc| (scales are under twice the
+// maxima), so one real dot product errs by at most that times K (S = 7:
+// 2^-44 K, typically far less as the dropped digits are not aligned); an
+// FP64 dot product has 2^-53 K sum|a||b| as its bound.  The complex
+// product uses the 3M form as the FP64 kernel does: T1 = Ar Br, T2 = Ai Bi,
+// T3 = (Ar + Ai)(Br + Bi), C = (T1 - T2) + i (T3 - T1 - T2).
+//
+// Data flow per launch: two slicing kernels write the digits of op(A) and
+// op(B) (int8, row-major along k, k padded to a multiple of 64 and rows to a
+// multiple of 64 with zeros) and their scales into the caller's workspace;
+// an operand with a zero batch stride is sliced once.  The product kernel
+// stages 64 x 64 digit tiles of all S digits of op(A) and op(B) through two
+// LDS buffers (one barrier per 64-wide k chunk), runs the three real products
+// one after another over k, and folds each into double accumulators.
+//
+// int8 MFMA 16x16x64 operands: lane l supplies 16 consecutive k of row l % 16
+// (A) or column l % 16 (B) starting at k = 16 (l / 16).  Any assignment of k to
+// (lane group, byte) gives the same sum as long as A and B use the same one.
+// Accumulator item v of lane l holds D[4 (l / 16) + v][l % 16].
+#include <cstdint>
+#include <initializer_list>
+
+#include "sinv_core.hpp"
+
+namespace sinv {
+
+namespace {
+
+typedef int i4v __attribute__((ext_vector_type(4)));
+
+constexpr int kTile = 64;   // workgroup tile (rows and columns)
+constexpr int kChunk = 64;  // k per LDS chunk (one MFMA's k)
+constexpr int kRowB = 80;   // LDS row stride in bytes (64 + 16 pad: conflict-free b128 reads)
+constexpr int kMaxKp = 16384;  // S * Kp * 127^2 must stay below 2^31
+
+inline long long round_up(long long x, long long m) { return (x + m - 1) / m * m; }
+
+// Which operand copies are distinct: bit 1 = inner stride non-zero, bit 0 =
+// outer stride non-zero.  The slice index zi of batch entry (z_in, z_out).
+__host__ __device__ inline int zmode(long long s_in, long long s_out) {
+  return (s_in != 0 ? 2 : 0) | (s_out != 0 ? 1 : 0);
+}
+__host__ __device__ inline long long zcount(int mode, int inner, int outer) {
+  return mode == 0 ? 1 : mode == 1 ? outer : mode == 2 ? inner : (long long)inner * outer;
+}
+__host__ __device__ inline long long zindex(int mode, int z_in, int z_out, int inner) {
+  return mode == 0 ? 0 : mode == 1 ? z_out : mode == 2 ? z_in : (long long)z_out * inner + z_in;
+}
+
+// op(X) seen as R "rows" of K elements: element (r, k) at X + r*rs + k*ks,
+// imaginary part negated when conj.  Digits of distinct copy zi, part p,
+// digit t at data + ((zi*3 + p)*S + t)*Rp*Kp + r*Kp + k; scales at
+// scale + (zi*3 + p)*Rp + r (2^e, 0 for an all-zero row).
+struct SliceArgs {
+  const cplx* X;
+  long long s_in, s_out;
+  int inner, mode;
+  int R, K, Rp, Kp;
+  long long rs, ks;
+  double conj;
+  int8_t* data;
+  double* scale;
+};
+
+__device__ __forceinline__ void parts_of(const cplx& x, double conj, double v[3]) {
+  const double im = conj * x.im;
+  v[0] = x.re;
+  v[1] = im;
+  v[2] = x.re + im;
+}
+
+__device__ __forceinline__ const cplx* slice_src(const SliceArgs& a, long long zi) {
+  long long z_in = 0, z_out = 0;
+  if (a.mode == 1) z_out = zi;
+  else if (a.mode == 2) z_in = zi;
+  else if (a.mode == 3) { z_out = zi / a.inner; z_in = zi - z_out * a.inner; }
+  return a.X + z_in * a.s_in + z_out * a.s_out;
+}
+
+// scale of a row from its maximum magnitude: 2^e with max < 2^e (frexp gives
+// max = f 2^e, f in [0.5, 1)); inv = 2^-e.  An all-zero row gets 0 and 0.
+__device__ __forceinline__ void scale_of(double mx, double& sc, double& inv) {
+  if (mx > 0.0) {
+    int e;
+    (void)frexp(mx, &e);
+    sc = ldexp(1.0, e);
+    inv = ldexp(1.0, -e);
+  } else {
+    sc = 0.0;
+    inv = 0.0;
+  }
+}
+
+// S digits of y in (-1, 1): exact in double (scaling by 128 and removing the
+// integer part lose nothing), each digit in [-127, 127].
+template <int S>
+__device__ __forceinline__ void digits(double y, int q[S]) {
+#pragma unroll
+  for (int t = 0; t < S; ++t) {
+    y *= 128.0;
+    const double d = trunc(y);
+    q[t] = (int)d;
+    y -= d;
+  }
+}
+
+__device__ __forceinline__ unsigned pack4(int a, int b, int c, int d) {
+  return (unsigned)(a & 0xff) | ((unsigned)(b & 0xff) << 8) | ((unsigned)(c & 0xff) << 16) |
+         ((unsigned)(d & 0xff) << 24);
+}
+
+// k-contiguous operand (ks == 1): one wave per row, lane l takes k in
+// [4l + 256j, 4l + 256j + 4).  grid (Rp / 4, distinct copies), 256 threads.
+template <int S>
+__global__ __launch_bounds__(256) void slice_rows_kernel(SliceArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long long zi = blockIdx.y;
+  const cplx* X = slice_src(a, zi);
+  const size_t plane = (size_t)a.Rp * a.Kp;
+  int8_t* base = a.data + (size_t)zi * 3 * S * plane + (size_t)r * a.Kp;
+  if (r >= a.R) {
+    for (int k = 4 * lane; k < a.Kp; k += 256)
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int t = 0; t < S; ++t)
+          *reinterpret_cast<unsigned*>(base + (size_t)(p * S + t) * plane + k) = 0u;
+    if (lane < 3) a.scale[(zi * 3 + lane) * a.Rp + r] = 0.0;
+    return;
+  }
+  const cplx* row = X + (long long)r * a.rs;
+  double mx[3] = {0.0, 0.0, 0.0};
+  for (int k0 = 4 * lane; k0 < a.K; k0 += 256) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = k0 + q;
+      if (k < a.K) {
+        double v[3];
+        parts_of(row[k], a.conj, v);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) mx[p] = fmax(mx[p], fabs(v[p]));
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int p = 0; p < 3; ++p) mx[p] = fmax(mx[p], __shfl_xor(mx[p], off));
+  double inv[3];
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    double sc;
+    scale_of(mx[p], sc, inv[p]);
+    if (lane == 0) a.scale[(zi * 3 + p) * a.Rp + r] = sc;
+  }
+  for (int k0 = 4 * lane; k0 < a.Kp; k0 += 256) {
+    int q[3][4][S];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = k0 + j;
+      double v[3] = {0.0, 0.0, 0.0};
+      if (k < a.K) parts_of(row[k], a.conj, v);
+#pragma unroll
+      for (int p = 0; p < 3; ++p) digits<S>(v[p] * inv[p], q[p][j]);
+    }
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int t = 0; t < S; ++t)
+        *reinterpret_cast<unsigned*>(base + (size_t)(p * S + t) * plane + k0) =
+            pack4(q[p][0][t], q[p][1][t], q[p][2][t], q[p][3][t]);
+  }
+}
+
+// row-contiguous operand (rs == 1), two passes over 64 x 64 pieces (rows
+// 64 blockIdx.x + lane, k in 64 blockIdx.y + [16 w, 16 w + 16) for wave w),
+// grid (Rp / 64, Kp / 64, distinct copies), 256 threads:
+//  1. colmax: the piece's row maxima per part -> pmax[(zi*nkc + kc)*3 + p][r]
+//  2. slice: each row's scale from its nkc partial maxima, then the piece's
+//     digits (16 consecutive k per thread, one 16-byte store per plane).
+__device__ __forceinline__ double* pmax_at(double* pmax, long long zi, int nkc, int kc, int p,
+                                           int Rp) {
+  return pmax + ((zi * nkc + kc) * 3 + p) * (long long)Rp;
+}
+
+__global__ __launch_bounds__(256) void colmax_kernel(SliceArgs a, double* pmax, int nkc) {
+  __shared__ double part[4][3][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + lane;
+  const int kc = blockIdx.y;
+  const long long zi = blockIdx.z;
+  const cplx* X = slice_src(a, zi);
+  double mx[3] = {0.0, 0.0, 0.0};
+  if (r < a.R) {
+#pragma unroll 4
+    for (int j = 0; j < 16; ++j) {
+      const int k = kc * 64 + w * 16 + j;
+      if (k < a.K) {
+        double v[3];
+        parts_of(X[(long long)r + (long long)k * a.ks], a.conj, v);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) mx[p] = fmax(mx[p], fabs(v[p]));
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 3; ++p) part[w][p][lane] = mx[p];
+  __syncthreads();
+  if (w < 3) {
+    const double m = fmax(fmax(part[0][w][lane], part[1][w][lane]),
+                          fmax(part[2][w][lane], part[3][w][lane]));
+    pmax_at(pmax, zi, nkc, kc, w, a.Rp)[r] = m;
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void slice_cols_kernel(SliceArgs a, const double* pmax,
+                                                         int nkc) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + lane;
+  const int kc = blockIdx.y;
+  const long long zi = blockIdx.z;
+  const cplx* X = slice_src(a, zi);
+  const size_t plane = (size_t)a.Rp * a.Kp;
+  double inv[3];
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    double m = 0.0;
+    for (int c = 0; c < nkc; ++c)
+      m = fmax(m, pmax_at(const_cast<double*>(pmax), zi, nkc, c, p, a.Rp)[r]);
+    double sc;
+    scale_of(m, sc, inv[p]);
+    if (kc == 0 && w == 0) a.scale[(zi * 3 + p) * a.Rp + r] = sc;
+  }
+  const bool live = r < a.R;
+  const int kg = kc * 4 + w;  // 16-wide k group
+  unsigned words[3][S][4];
+#pragma unroll
+  for (int j4 = 0; j4 < 4; ++j4) {
+    int q[3][4][S];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = kg * 16 + j4 * 4 + j;
+      double v[3] = {0.0, 0.0, 0.0};
+      if (live && k < a.K) parts_of(X[(long long)r + (long long)k * a.ks], a.conj, v);
+#pragma unroll
+      for (int p = 0; p < 3; ++p) digits<S>(v[p] * inv[p], q[p][j]);
+    }
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int t = 0; t < S; ++t)
+        words[p][t][j4] = pack4(q[p][0][t], q[p][1][t], q[p][2][t], q[p][3][t]);
+  }
+  int8_t* base = a.data + (size_t)zi * 3 * S * plane + (size_t)r * a.Kp + kg * 16;
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      i4v v4;
+      v4.x = (int)words[p][t][0];
+      v4.y = (int)words[p][t][1];
+      v4.z = (int)words[p][t][2];
+      v4.w = (int)words[p][t][3];
+      *reinterpret_cast<i4v*>(base + (size_t)(p * S + t) * plane) = v4;
+    }
+}
+
+struct ProdArgs {
+  const int8_t* a_data;
+  const double* a_scale;
+  int a_mode, a_Rp;
+  const int8_t* b_data;
+  const double* b_scale;
+  int b_mode, b_Rp;
+  int Kp;
+};
+
+// product kernel: 256 threads = 4 waves (2 x 2), workgroup tile 64 x 64,
+// wave tile 32 x 32 = 2 x 2 MFMA tiles; S digit planes per operand.
+template <int S>
+__global__ __launch_bounds__(256) void gemm_i8x_kernel(GemmArgs g, ProdArgs o, int tiles_m,
+                                                       int tiles_n) {
+  __shared__ __attribute__((aligned(16))) int8_t lds[2][2][S][kTile][kRowB];
+
+  // XCD-aware order: workgroup bid runs on XCD bid % 8; give each XCD a
+  // contiguous range of tiles (neighbouring batch entries share its L2)
+  int bid = blockIdx.x;
+  const int total = gridDim.x;
+  if ((total & 7) == 0) bid = (bid & 7) * (total >> 3) + (bid >> 3);
+  const int tiles = tiles_m * tiles_n;
+  const int z = bid / tiles;
+  const int t = bid - z * tiles;
+  const int z_out = z / g.inner, z_in = z - z_out * g.inner;
+  const int m0 = (t / tiles_n) * kTile, n0 = (t % tiles_n) * kTile;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int Kp = o.Kp, nk = Kp / kChunk, nc = 3 * nk;
+
+  const long long za = zindex(o.a_mode, z_in, z_out, g.inner);
+  const long long zb = zindex(o.b_mode, z_in, z_out, g.inner);
+  const size_t a_plane = (size_t)o.a_Rp * Kp, b_plane = (size_t)o.b_Rp * Kp;
+  // this thread's 16-byte piece of every digit tile: row tid / 4, bytes 16 (tid % 4)
+  const int lr = tid >> 2, lc = (tid & 3) * 16;
+  const int8_t* a_src = o.a_data + (size_t)za * 3 * S * a_plane + (size_t)(m0 + lr) * Kp + lc;
+  const int8_t* b_src = o.b_data + (size_t)zb * 3 * S * b_plane + (size_t)(n0 + lr) * Kp + lc;
+
+  i4v ra[S], rb[S];
+  auto load = [&](int c) {
+    const int p = c / nk, kc = c - p * nk;
+    const size_t ka = (size_t)p * S * a_plane + (size_t)kc * kChunk;
+    const size_t kb = (size_t)p * S * b_plane + (size_t)kc * kChunk;
+#pragma unroll
+    for (int d = 0; d < S; ++d) {
+      ra[d] = *reinterpret_cast<const i4v*>(a_src + ka + d * a_plane);
+      rb[d] = *reinterpret_cast<const i4v*>(b_src + kb + d * b_plane);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int d = 0; d < S; ++d) {
+      *reinterpret_cast<i4v*>(&lds[buf][0][d][lr][lc]) = ra[d];
+      *reinterpret_cast<i4v*>(&lds[buf][1][d][lr][lc]) = rb[d];
+    }
+  };
+
+  i4v acc[2][2][S];
+  double cr[2][2][4], ci[2][2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+      for (int d = 0; d < S; ++d) acc[i][j][d] = i4v{0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 4; ++v) cr[i][j][v] = ci[i][j][v] = 0.0;
+    }
+
+  const int fr = lane & 15, fk = (lane >> 4) * 16;
+  load(0);
+  store(0);
+  __syncthreads();
+  // the three real products one after another; accumulators live only
+  // inside one product's k loop
+#pragma unroll 1
+  for (int p = 0; p < 3; ++p) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int d = 0; d < S; ++d) acc[i][j][d] = i4v{0, 0, 0, 0};
+#pragma unroll 1
+    for (int kc = 0; kc < nk; ++kc) {
+      const int c = p * nk + kc;
+      const int buf = c & 1;
+      if (c + 1 < nc) load(c + 1);
+      i4v bf[2][S];
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int u = 0; u < S; ++u)
+          bf[ct][u] =
+              *reinterpret_cast<const i4v*>(&lds[buf][1][u][wn * 32 + ct * 16 + fr][fk]);
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ta = 0; ta < S; ++ta) {
+          // one digit plane of op(A) at a time, against every B digit it pairs with
+          const i4v af =
+              *reinterpret_cast<const i4v*>(&lds[buf][0][ta][wm * 32 + rt * 16 + fr][fk]);
+#pragma unroll
+          for (int ub = 0; ta + ub < S; ++ub)
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+              acc[rt][ct][ta + ub] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                  af, bf[ct][ub], acc[rt][ct][ta + ub], 0, 0, 0);
+        }
+      if (c + 1 < nc) store(buf ^ 1);
+      __syncthreads();
+    }
+    // fold real product p into the complex accumulators (3M combination)
+    const double* sa = o.a_scale + ((size_t)za * 3 + p) * o.a_Rp + m0 + wm * 32;
+    const double* sb = o.b_scale + ((size_t)zb * 3 + p) * o.b_Rp + n0 + wn * 32;
+    const double sr = p == 0 ? 1.0 : -1.0;  // sign of T_p in the real part (T3: 0)
+    const double si = p == 2 ? 1.0 : -1.0;  // sign of T_p in the imaginary part
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        const double scol = sb[ct * 16 + fr];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double val = 0.0;
+#pragma unroll
+          for (int d = S - 1; d >= 0; --d)
+            val += (double)acc[rt][ct][d][v] * __builtin_ldexp(1.0, -7 * (d + 2));
+          val *= sa[rt * 16 + (lane >> 4) * 4 + v] * scol;
+          if (p < 2) cr[rt][ct][v] += sr * val;
+          ci[rt][ct][v] += si * val;
+        }
+      }
+  }
+
+  // epilogue: C = alpha T + beta C0 + shift I, in bounds only
+  cplx* C = g.C + z_in * g.sC_in + z_out * g.sC_out;
+  const cplx* C0 = g.C0 ? g.C0 + z_in * g.sC0_in + z_out * g.sC0_out : nullptr;
+  const cplx sh = g.shift ? g.shift[z_out] : cplx{0.0, 0.0};
+#pragma unroll
+  for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = m0 + wm * 32 + rt * 16 + (lane >> 4) * 4 + v;
+        const int col = n0 + wn * 32 + ct * 16 + fr;
+        if (row < g.M && col < g.N) {
+          const double tr = cr[rt][ct][v], ti = ci[rt][ct][v];
+          double re = g.alpha.re * tr - g.alpha.im * ti;
+          double im = g.alpha.re * ti + g.alpha.im * tr;
+          if (C0) {
+            const cplx c0 = C0[(long long)row * g.ldc0 + col];
+            re += g.beta.re * c0.re - g.beta.im * c0.im;
+            im += g.beta.re * c0.im + g.beta.im * c0.re;
+          }
+          if (row == col) {
+            re += sh.re;
+            im += sh.im;
+          }
+          C[(long long)row * g.ldc + col] = cplx{re, im};
+        }
+      }
+}
+
+struct I8xPlan {
+  int a_mode, b_mode;
+  long long a_nz, b_nz;
+  int Mp, Np, Kp;
+  size_t a_data, a_scale, b_data, b_scale, pmax, total;  // byte offsets / total
+};
+
+inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+I8xPlan plan_i8x(const GemmArgs& g, int S) {
+  I8xPlan p{};
+  p.a_mode = zmode(g.sA_in, g.sA_out);
+  p.b_mode = zmode(g.sB_in, g.sB_out);
+  p.a_nz = zcount(p.a_mode, g.inner, g.outer);
+  p.b_nz = zcount(p.b_mode, g.inner, g.outer);
+  p.Mp = (int)round_up(g.M, kTile);
+  p.Np = (int)round_up(g.N, kTile);
+  p.Kp = (int)round_up(g.K, kChunk);
+  size_t off = 0;
+  p.a_data = off;
+  off = al256(off + (size_t)p.a_nz * 3 * S * p.Mp * p.Kp);
+  p.a_scale = off;
+  off = al256(off + (size_t)p.a_nz * 3 * p.Mp * sizeof(double));
+  p.b_data = off;
+  off = al256(off + (size_t)p.b_nz * 3 * S * p.Np * p.Kp);
+  p.b_scale = off;
+  off = al256(off + (size_t)p.b_nz * 3 * p.Np * sizeof(double));
+  // partial row maxima of a row-contiguous operand (one per 64-wide k chunk)
+  const size_t nkc = (size_t)p.Kp / kChunk;
+  const size_t pa = (size_t)p.a_nz * nkc * 3 * p.Mp, pb = (size_t)p.b_nz * nkc * 3 * p.Np;
+  p.pmax = off;
+  off = al256(off + (pa > pb ? pa : pb) * sizeof(double));
+  p.total = off + 256;  // room to align the base
+  return p;
+}
+
+template <int S>
+hipError_t launch_i8x_s(const GemmArgs& g, const I8xPlan& p, uint8_t* base, hipStream_t s) {
+  // op(A): rows m, k along K;  element (m, k) = A[m*lda + k] (N) or A[k*lda + m] (T, C)
+  SliceArgs sa{};
+  sa.X = g.A; sa.s_in = g.sA_in; sa.s_out = g.sA_out; sa.inner = g.inner; sa.mode = p.a_mode;
+  sa.R = g.M; sa.K = g.K; sa.Rp = p.Mp; sa.Kp = p.Kp;
+  sa.rs = g.opA == OP_N ? g.lda : 1; sa.ks = g.opA == OP_N ? 1 : g.lda;
+  sa.conj = g.opA == OP_C ? -1.0 : 1.0;
+  sa.data = reinterpret_cast<int8_t*>(base + p.a_data);
+  sa.scale = reinterpret_cast<double*>(base + p.a_scale);
+  // op(B): rows n (columns of op(B)); element (n, k) = B[k*ldb + n] (N) or B[n*ldb + k] (T, C)
+  SliceArgs sb{};
+  sb.X = g.B; sb.s_in = g.sB_in; sb.s_out = g.sB_out; sb.inner = g.inner; sb.mode = p.b_mode;
+  sb.R = g.N; sb.K = g.K; sb.Rp = p.Np; sb.Kp = p.Kp;
+  sb.rs = g.opB == OP_N ? 1 : g.ldb; sb.ks = g.opB == OP_N ? g.ldb : 1;
+  sb.conj = g.opB == OP_C ? -1.0 : 1.0;
+  sb.data = reinterpret_cast<int8_t*>(base + p.b_data);
+  sb.scale = reinterpret_cast<double*>(base + p.b_scale);
+
+  const int nkc = p.Kp / kChunk;
+  double* pmax = reinterpret_cast<double*>(base + p.pmax);
+  for (const SliceArgs* a : {&sa, &sb}) {
+    const unsigned nz = (unsigned)(a == &sa ? p.a_nz : p.b_nz);
+    if (a->ks == 1) {
+      hipLaunchKernelGGL(slice_rows_kernel<S>, dim3((unsigned)(a->Rp / 4), nz), dim3(256), 0, s,
+                         *a);
+    } else {
+      // the two operands' column passes run one after the other on the stream,
+      // so they share the partial-maximum buffer
+      const dim3 grid((unsigned)(a->Rp / 64), (unsigned)nkc, nz);
+      hipLaunchKernelGGL(colmax_kernel, grid, dim3(256), 0, s, *a, pmax, nkc);
+      SINV_HIP_RET(hipGetLastError());
+      hipLaunchKernelGGL(slice_cols_kernel<S>, grid, dim3(256), 0, s, *a, pmax, nkc);
+    }
+    SINV_HIP_RET(hipGetLastError());
+  }
+
+  ProdArgs o{};
+  o.a_data = sa.data; o.a_scale = sa.scale; o.a_mode = p.a_mode; o.a_Rp = p.Mp;
+  o.b_data = sb.data; o.b_scale = sb.scale; o.b_mode = p.b_mode; o.b_Rp = p.Np;
+  o.Kp = p.Kp;
+  const int tm = p.Mp / kTile, tn = p.Np / kTile;
+  const long long blocks = (long long)tm * tn * g.inner * g.outer;
+  hipLaunchKernelGGL(gemm_i8x_kernel<S>, dim3((unsigned)blocks), dim3(256), 0, s, g, o, tm, tn);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+bool i8x_slices_ok(int S) { return S >= kI8xMinSlices && S <= kI8xMaxSlices; }
+
+size_t gemm_i8x_workspace_bytes(const GemmArgs& g, int S) {
+  if (!i8x_slices_ok(S) || g.M <= 0 || g.N <= 0 || g.K <= 0 || g.inner <= 0 || g.outer <= 0)
+    return 0;
+  return plan_i8x(g, S).total;
+}
+
+hipError_t launch_gemm_i8x(const GemmArgs& g, int S, void* work, size_t work_bytes,
+                           hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0 || g.inner <= 0 || g.outer <= 0) return hipSuccess;
+  if (g.K <= 0) return launch_gemm(g, Prec::F64, s);  // no product term: C = beta C0 + shift
+  if (!i8x_slices_ok(S)) return hipErrorInvalidValue;
+  const I8xPlan p = plan_i8x(g, S);
+  if (p.Kp > kMaxKp) return hipErrorInvalidValue;
+  if (work == nullptr || work_bytes < p.total) return hipErrorInvalidValue;
+  const long long blocks = (long long)(p.Mp / kTile) * (p.Np / kTile) * g.inner * g.outer;
+  if (blocks > 0x7fffffffLL || p.a_nz > 65535 || p.b_nz > 65535)
+    return hipErrorInvalidConfiguration;
+  uint8_t* base = (uint8_t*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
+  const int pid = prof_open(s);
+  hipError_t e;
+  switch (S) {
+    case 5: e = launch_i8x_s<5>(g, p, base, s); break;
+    case 6: e = launch_i8x_s<6>(g, p, base, s); break;
+    default: e = launch_i8x_s<7>(g, p, base, s); break;
+  }
+  if (e != hipSuccess) return e;
+  const double nb = (double)g.inner * g.outer;
+  prof_shape(pid, g.M, g.N, g.K, (long long)nb, g.opA | (g.opB << 2) | (g.C0 != nullptr ? 16 : 0));
+  const double elems = (double)g.M * g.K + (double)g.K * g.N +
+                       (double)g.M * g.N * (g.C0 != nullptr ? 2.0 : 1.0);
+  prof_close(pid, s, PROF_GEMM_I8X, 8.0 * g.M * (double)g.N * g.K * nb, sizeof(cplx) * elems * nb);
+  return hipSuccess;
+}
+
+}  // namespace sinv
+
+using namespace sinv;
+
+extern "C" size_t sinv_zgemm_int8x_workspace_bytes(int M, int N, int K, long long strideA,
+                                                   long long strideB, int batch, int slices) {
+  GemmArgs g{};
+  g.M = M; g.N = N; g.K = K;
+  g.sA_out = strideA; g.sB_out = strideB;
+  g.inner = 1; g.outer = batch;
+  return gemm_i8x_workspace_bytes(g, slices);
+}
+
+extern "C" int sinv_zgemm_int8x_batched(int M, int N, int K, int opA, int opB, sinv_c128 alpha,
+                                        const void* A, int lda, long long strideA, const void* B,
+                                        int ldb, long long strideB, sinv_c128 beta,
+                                        const void* C0, int ldc0, long long strideC0, void* C,
+                                        int ldc, long long strideC, const void* shift, int batch,
+                                        int slices, void* work, size_t work_bytes, void* stream) {
+  if (M < 0 || N < 0 || K < 0 || batch < 0 || opA < 0 || opA > 2 || opB < 0 || opB > 2 ||
+      !i8x_slices_ok(slices) || strideA < 0 || strideB < 0 || strideC < 0 || strideC0 < 0) {
+    set_error("sinv_zgemm_int8x_batched: invalid argument");
+    return SINV_E_INVAL;
+  }
+  if (M == 0 || N == 0 || batch == 0) return SINV_OK;
+  if (A == nullptr || B == nullptr || C == nullptr || (K == 0 && C0 == nullptr)) {
+    set_error("sinv_zgemm_int8x_batched: null operand");
+    return SINV_E_INVAL;
+  }
+  const int ra = opA == OP_N ? K : M, rb = opB == OP_N ? N : K;
+  if (lda < (ra > 1 ? ra : 1) || ldb < (rb > 1 ? rb : 1) || ldc < N || (C0 && ldc0 < N)) {
+    set_error("sinv_zgemm_int8x_batched: leading dimension too small");
+    return SINV_E_INVAL;
+  }
+  if ((long long)K > kMaxKp) {
+    set_error("sinv_zgemm_int8x_batched: K = %d exceeds %d (int32 digit sums)", K, kMaxKp);
+    return SINV_E_INVAL;
+  }
+  GemmArgs g{};
+  g.M = M; g.N = N; g.K = K; g.opA = opA; g.opB = opB;
+  g.A = (const cplx*)A; g.lda = lda; g.sA_out = strideA;
+  g.B = (const cplx*)B; g.ldb = ldb; g.sB_out = strideB;
+  g.C0 = (const cplx*)C0; g.ldc0 = ldc0; g.sC0_out = strideC0;
+  g.C = (cplx*)C; g.ldc = ldc; g.sC_out = strideC;
+  g.alpha = cplx{alpha.re, alpha.im};
+  g.beta = cplx{beta.re, beta.im};
+  g.shift = (const cplx*)shift;
+  g.inner = 1; g.outer = batch;
+  const size_t need = K > 0 ? gemm_i8x_workspace_bytes(g, slices) : 0;
+  if (K > 0 && (work == nullptr || work_bytes < need)) {
+    set_error("sinv_zgemm_int8x_batched: workspace too small (%zu < %zu)", work_bytes, need);
+    return SINV_E_INVAL;
+  }
+  SINV_HIP_TRY(launch_gemm_i8x(g, slices, work, work_bytes, (hipStream_t)stream));
+  return SINV_OK;
+}

@@ -34,10 +34,11 @@ Layout make_layout(int N, int blocksize, const int* bsizes) {
 struct WorkPlan {
   size_t tmp_elems = 0;  // per-instance temporaries P, Q, X (each bmax^2)
   size_t inv_elems = 0;  // inverse workspace for the whole batch
+  size_t i8x_bytes = 0;  // digit planes of the sliced-int8 GEMMs (SINV_PREC_INT8X)
   size_t total_bytes = 0;
 };
 
-WorkPlan plan_work(const Layout& L, int batch) {
+WorkPlan plan_work(const Layout& L, int batch, int precision) {
   WorkPlan w;
   const size_t b2 = (size_t)L.bmax * L.bmax;
   w.tmp_elems = 3 * b2 * (size_t)batch;
@@ -47,7 +48,16 @@ WorkPlan plan_work(const Layout& L, int batch) {
     if (e > inv) inv = e;
   }
   w.inv_elems = inv;
-  w.total_bytes = (w.tmp_elems + w.inv_elems) * sizeof(cplx) + 256;
+  if (precision == SINV_PREC_INT8X) {
+    // largest sweep product: bmax^3 with both operands distinct per instance
+    GemmArgs g{};
+    g.M = g.N = g.K = L.bmax;
+    g.sA_out = g.sB_out = 1;
+    g.inner = 1;
+    g.outer = batch;
+    w.i8x_bytes = gemm_i8x_workspace_bytes(g, kI8xDefaultSlices);
+  }
+  w.total_bytes = (w.tmp_elems + w.inv_elems) * sizeof(cplx) + w.i8x_bytes + 256;
   return w;
 }
 
@@ -55,7 +65,9 @@ bool desc_valid(const sinv_rgf_desc* d) {
   if (d == nullptr) return false;
   if (d->nblocks <= 0 || d->batch <= 0) return false;
   if (d->bsizes == nullptr && d->blocksize <= 0) return false;
-  if (d->precision != SINV_PREC_FP64 && d->precision != SINV_PREC_MIXED) return false;
+  if (d->precision != SINV_PREC_FP64 && d->precision != SINV_PREC_MIXED &&
+      d->precision != SINV_PREC_INT8X)
+    return false;
   if (d->diag == nullptr || d->g_diag == nullptr) return false;
   if (d->nblocks > 1 && (d->upper == nullptr || d->lower == nullptr)) return false;
   if (d->nblocks > 1 && d->save_off_diag && (d->g_upper == nullptr || d->g_lower == nullptr))
@@ -87,7 +99,7 @@ extern "C" size_t sinv_rgf_workspace_bytes(const sinv_rgf_desc* d) {
   if (!desc_valid(d)) return 0;
   const Layout L = make_layout(d->nblocks, d->blocksize, d->bsizes);
   if (!L.ok) return 0;
-  return plan_work(L, d->batch).total_bytes;
+  return plan_work(L, d->batch, d->precision).total_bytes;
 }
 
 extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_bytes,
@@ -101,14 +113,15 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
     set_error("sinv_rgf_batched: invalid block sizes");
     return SINV_E_INVAL;
   }
-  const WorkPlan wp = plan_work(L, d->batch);
+  const WorkPlan wp = plan_work(L, d->batch, d->precision);
   if (work == nullptr || work_bytes < wp.total_bytes) {
     set_error("sinv_rgf_batched: workspace too small (%zu < %zu)", work_bytes, wp.total_bytes);
     return SINV_E_INVAL;
   }
   hipStream_t s = (hipStream_t)stream;
   const int N = L.N, nb = d->batch;
-  const Prec prec = d->precision == SINV_PREC_FP64 ? Prec::F64 : Prec::F32;
+  const Prec prec = d->precision == SINV_PREC_MIXED ? Prec::F32 : Prec::F64;
+  const bool i8x = d->precision == SINV_PREC_INT8X;
   const long long b2 = (long long)L.bmax * L.bmax;
 
   const cplx* D = (const cplx*)d->diag;
@@ -129,6 +142,7 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
   cplx* Q = P + b2 * nb;
   cplx* Xw = Q + b2 * nb;
   cplx* winv = Xw + b2 * nb;
+  void* wi8x = winv + wp.inv_elems;
 
   // one GEMM launch over all instances: instance z is the outer batch index
   auto gemm = [&](int M, int Nn, int K, const cplx* A, int lda, long long sA, const cplx* B,
@@ -143,6 +157,7 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
     g.C = C; g.ldc = ldc; g.sC_in = 0; g.sC_out = sC;
     g.alpha = alpha; g.beta = beta; g.shift = sh;
     g.inner = 1; g.outer = nb;
+    if (i8x) return launch_gemm_i8x(g, kI8xDefaultSlices, wi8x, wp.i8x_bytes, s);
     return launch_gemm(g, p, s);
   };
 

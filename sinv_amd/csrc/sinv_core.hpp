@@ -43,6 +43,15 @@ enum class Prec : int { F64 = 0, F32 = 1 };
 // K2: batched complex GEMM on MFMA (FP64 16x16x4 for F64, FP32 16x16x4 for F32).
 hipError_t launch_gemm(const GemmArgs& g, Prec p, hipStream_t s);
 
+// K2 in FP64-accurate sliced-int8 form (int8x.hip): the same product from S
+// 7-bit digit planes per operand on the int8 matrix cores, exact int32 sums.
+// The workspace holds the digit planes and row scales of both operands.
+constexpr int kI8xMinSlices = 5, kI8xMaxSlices = 7, kI8xDefaultSlices = 7;
+bool i8x_slices_ok(int S);
+size_t gemm_i8x_workspace_bytes(const GemmArgs& g, int S);
+hipError_t launch_gemm_i8x(const GemmArgs& g, int S, void* work, size_t work_bytes,
+                           hipStream_t s);
+
 // C[z] = a * D[z] + shift[z_outer] * I  (elementwise, b x b blocks), batched as GemmArgs.
 hipError_t launch_axpby_shift(int M, int N, cplx a, const cplx* D, int ldd, long long sD_in,
                               long long sD_out, cplx* C, int ldc, long long sC_in,
@@ -142,7 +151,13 @@ void set_error(const char* fmt, ...);
 // launch profiler (prof.hip): prof_open before a launch returns an id (-1 when
 // off), prof_close after it records the kernel kind and its algorithmic flops
 // and bytes (the operands read and results written once each, as stored).
-enum ProfKind : int { PROF_GEMM_F64 = 0, PROF_GEMM_F32 = 1, PROF_INV_LEAF = 2, PROF_OTHER = 3 };
+enum ProfKind : int {
+  PROF_GEMM_F64 = 0,
+  PROF_GEMM_F32 = 1,
+  PROF_INV_LEAF = 2,
+  PROF_OTHER = 3,
+  PROF_GEMM_I8X = 4
+};
 int prof_open(hipStream_t s);
 void prof_close(int id, hipStream_t s, int kind, double flops, double bytes);
 // shape of a profiled GEMM launch (kept in the per-launch records)

@@ -1,0 +1,165 @@
+"""GPU tests of the sliced-int8 complex GEMM (sinv_zgemm_int8x_batched) and of
+RGF with precision="int8x".
+
+The product is checked against numpy's complex128 matmul with the error bound
+the digit splitting gives: per term, the dropped digit pairs (S - 1 of size
+2^(-7S) at most, then geometrically smaller) and the two remainders beyond S
+digits give 4 (S + 1) 2^(-7S) max|row of op(A)| max|column of op(B)| (the
+scales are below twice the maxima); K terms, and the 3M complex form adds at
+most three real products per part.  RGF in int8x mode is checked against the FP64 path and the
+oracle at the 1e-8 gate.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import dense, rgf_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _op(X, op):
+    return X if op == "N" else (X.T if op == "T" else X.conj().T)
+
+
+def _bound(A, B, opA, opB, S):
+    """Elementwise bound (M x N) of |int8x - exact| for one batch entry."""
+    a = np.abs(_op(A, opA))
+    b = np.abs(_op(B, opB))
+    # the three real parts of a row are bounded by 2 max|x|
+    ra = 2 * a.max(axis=1)[:, None]
+    cb = 2 * b.max(axis=0)[None, :]
+    K = a.shape[1]
+    return 3 * (4 * (S + 1) * 2.0 ** (-7 * S) + 1e-15) * K * ra * cb
+
+
+@pytest.mark.parametrize("M,N,K", [(64, 64, 64), (100, 70, 130), (1, 5, 3), (256, 128, 384),
+                                   (416, 256, 416), (33, 200, 65)])
+@pytest.mark.parametrize("ops", [("N", "N"), ("T", "N"), ("N", "C"), ("C", "T")])
+def test_zgemm_int8x_against_fp64(built_lib, M, N, K, ops):
+    import torch
+    from sinv_amd import core
+    opA, opB = ops
+    print(f"int8x gemm M={M} N={N} K={K} ops={ops}", flush=True)
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    batch = 3
+    A = _rand(rng, (batch,) + ((M, K) if opA == "N" else (K, M)))
+    B = _rand(rng, (1,) + ((K, N) if opB == "N" else (N, K)))  # shared by the batch
+    C0 = _rand(rng, (batch, M, N))
+    shift = _rand(rng, (batch,))
+    alpha, beta = 0.7 - 0.2j, -1.3 + 0.4j
+    C = core.zgemm_batched(torch.from_numpy(A), torch.from_numpy(B), opA=opA, opB=opB,
+                           alpha=alpha, beta=beta, C0=torch.from_numpy(C0),
+                           shift=torch.from_numpy(shift), precision="int8x").cpu().numpy()
+    for z in range(batch):
+        ref = alpha * (_op(A[z], opA) @ _op(B[0], opB)) + beta * C0[z]
+        ref[np.arange(min(M, N)), np.arange(min(M, N))] += shift[z]
+        err = np.abs(C[z] - ref)
+        bound = abs(alpha) * _bound(A[z], B[0], opA, opB, 7) + 1e-14 * np.abs(ref)
+        assert np.all(err <= bound), (z, float((err / bound).max()))
+        rel = np.linalg.norm(C[z] - ref) / np.linalg.norm(ref)
+        assert rel < 1e-13, rel
+
+
+@pytest.mark.parametrize("S", [5, 6, 7])
+def test_zgemm_int8x_digits_set_the_error(built_lib, S):
+    """Each extra digit plane cuts the error by about 2^7, within the bound."""
+    import torch
+    from sinv_amd import core
+    rng = np.random.default_rng(11)
+    M = N = K = 192
+    A = _rand(rng, (2, M, K))
+    B = _rand(rng, (2, K, N))
+    C = core.zgemm_batched(torch.from_numpy(A), torch.from_numpy(B), precision="int8x",
+                           slices=S).cpu().numpy()
+    ref = A @ B
+    for z in range(2):
+        err = np.abs(C[z] - ref[z])
+        assert np.all(err <= _bound(A[z], B[z], "N", "N", S))
+    rel = np.linalg.norm(C - ref) / np.linalg.norm(ref)
+    # observed scale of the error: 2^(-7S) times a modest factor
+    assert rel < 2.0 ** (-7 * S) * 64, (S, rel)
+
+
+def test_zgemm_int8x_row_scales(built_lib):
+    """Rows and columns spanning 2^-40 .. 2^40 keep their relative accuracy
+    (each gets its own power-of-two scale); zero rows give zero."""
+    import torch
+    from sinv_amd import core
+    rng = np.random.default_rng(5)
+    M, N, K = 96, 80, 112
+    A = _rand(rng, (M, K)) * (2.0 ** rng.integers(-40, 40, size=(M, 1)))
+    B = _rand(rng, (K, N)) * (2.0 ** rng.integers(-40, 40, size=(1, N)))
+    A[7] = 0
+    B[:, 3] = 0
+    C = core.zgemm_batched(torch.from_numpy(A), torch.from_numpy(B),
+                           precision="int8x").cpu().numpy()[0]
+    ref = A @ B
+    assert np.all(C[7] == 0) and np.all(C[:, 3] == 0)
+    assert np.all(np.abs(C - ref) <= _bound(A, B, "N", "N", 7))
+    rel_rows = np.linalg.norm(C - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-300)
+    assert rel_rows.max() < 1e-12
+
+
+def test_zgemm_int8x_rejects_bad_arguments(built_lib):
+    import torch
+    from sinv_amd import core, _native as nat
+    A = torch.zeros((1, 4, 4), dtype=torch.complex128)
+    with pytest.raises(ValueError):
+        core.zgemm_batched(A, A, precision="int8x", slices=4)
+    with pytest.raises(ValueError):
+        core.zgemm_batched(A, A, precision="int8x", slices=8)
+    # the FP64/complex64 entry point does not take the int8x code (it needs a workspace)
+    L = nat.lib()
+    rc = L.sinv_zgemm_batched(4, 4, 4, 0, 0, nat.C128.of(1), None, 4, 0, None, 4, 0,
+                              nat.C128.of(0), None, 4, 0, None, 4, 0, None, 1, 2, None)
+    assert rc == nat.SINV_E_INVAL
+
+
+def test_rgf_int8x_energies_meet_the_gate(built_lib):
+    """Batched energies near the band (eta = 1e-3): int8x RGF within 1e-8 of
+    the FP64 path and of the oracle, on every instance."""
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    H = configs.hermitian_H(12, 128, seed=3)
+    E = configs.energies(16, seed=4)
+    eta = 1e-3
+    print("rgf int8x energies N=12 b=128 E=16", flush=True)
+    G64 = core.rgf_energies(*H, E, eta)
+    G8 = core.rgf_energies(*H, E, eta, precision="int8x")
+    worst = 0.0
+    for k in range(len(E)):
+        a = tuple(x[k].cpu().numpy() for x in G8)
+        b = tuple(x[k].cpu().numpy() for x in G64)
+        worst = max(worst, dense.rel_err(a, b))
+    assert worst < 1e-8, worst
+    for k in (0, 7):
+        d, u, l = configs.shifted(H, E[k] + 1j * eta)
+        ref = rgf_ref.rgf_blocks(d, u, l)
+        got = tuple(x[k].cpu().numpy() for x in G8)
+        assert dense.rel_err(got, ref) < 1e-8
+
+
+def test_rgf_int8x_ragged_and_unsaved(built_lib):
+    """Per-block sizes (not multiples of 64) and save_off_diag=False in int8x mode."""
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    bs = [40, 96, 72, 130, 64, 17]
+    H = configs.hermitian_H_ragged(bs, seed=9)
+    d, u, l = configs.shifted_lists(H, 0.3 + 1e-2j)
+    G64 = core.rgf_ragged(d, u, l)
+    G8 = core.rgf_ragged(d, u, l, precision="int8x")
+    for x, y in zip(G8, G64):
+        for a, b in zip(x, y):
+            a = a.cpu().numpy() if hasattr(a, "cpu") else a
+            b = b.cpu().numpy() if hasattr(b, "cpu") else b
+            assert np.linalg.norm(a - b) <= 1e-10 * np.linalg.norm(b)
+    dd, uu, ll = configs.dominant_blocks(6, 80, seed=2)
+    Gd, Gu, Gl = core.rgf_batched(dd, uu, ll, save_off_diag=False, precision="int8x")
+    ref = rgf_ref.rgf_blocks(dd, uu, ll)
+    assert np.linalg.norm(Gd[0].cpu().numpy() - ref[0]) <= 1e-12 * np.linalg.norm(ref[0])
+    assert np.isnan(Gu.cpu().numpy()).all() and np.isnan(Gl.cpu().numpy()).all()
