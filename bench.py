@@ -602,8 +602,8 @@ def bench_adjoint(args):
 
     out = {}
     tuned = {}
-    for prec in ("fp64", "mixed"):
-        if not args.no_tune:
+    for prec in ("fp64", "mixed", "int8x"):
+        if not args.no_tune and prec != "int8x":
             # every chunk issues the same GEMM shapes: tune on one (untimed)
             from sinv_amd import autotune
             t0 = time.perf_counter()
@@ -631,10 +631,35 @@ def bench_adjoint(args):
             step(prec, limit=args.chunk)
             torch.cuda.synchronize()
         out[prec] = dict(ms=ms, loss=float(loss), grad=grad, prof=prof.stats, peak_gb=peak)
-    num = sum(float(((a - b).abs() ** 2).sum()) for a, b in zip(out["mixed"]["grad"], out["fp64"]["grad"]))
-    den = sum(float((b.abs() ** 2).sum()) for b in out["fp64"]["grad"])
-    gerr = float(np.sqrt(num / den))
-    lerr = abs(out["mixed"]["loss"] - out["fp64"]["loss"]) / abs(out["fp64"]["loss"])
+    def grad_err(prec):
+        num = sum(float(((a - b).abs() ** 2).sum()) for a, b in zip(out[prec]["grad"], out["fp64"]["grad"]))
+        den = sum(float((b.abs() ** 2).sum()) for b in out["fp64"]["grad"])
+        return (float(np.sqrt(num / den)),
+                abs(out[prec]["loss"] - out["fp64"]["loss"]) / abs(out["fp64"]["loss"]))
+
+    gerr, lerr = grad_err("mixed")
+    gerr8, lerr8 = grad_err("int8x")
+    log(f"[bench] gradient error vs fp64: mixed {gerr:.3e}, int8x {gerr8:.3e}")
+
+    # plain FP64: one energy per call (batch = 1), on a sample of energies
+    nplain = 3
+    for k in range(2):  # warm-up (first call allocates)
+        core.rgf_adjoint_ragged(Hd, Hu, Hl, gd, zu, zl, shift=E[k:k + 1], dscale=-1.0,
+                                oscale=-1.0, checkpoint_every=every, check=False,
+                                return_g_diag=True)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for k in range(nplain):
+        core.rgf_adjoint_ragged(Hd, Hu, Hl, gd, zu, zl, shift=E[k:k + 1], dscale=-1.0, oscale=-1.0,
+                                checkpoint_every=every, check=False, return_g_diag=True)
+    e1.record()
+    torch.cuda.synchronize()
+    plain_ms = e0.elapsed_time(e1) / nplain
+    plain = dict(solves_per_s=round(1e3 / plain_ms, 3), ms_per_solve=round(plain_ms, 3),
+                 sample=f"{nplain} energies, one forward+adjoint call each (batch = 1)")
+    log(f"[bench] plain fp64 batch=1 forward+adjoint: {plain_ms:.1f} ms/solve")
     fwd_flops = configs.rgf_flops_ragged(bs) * nE
     head = "fp64"
     ms = out[head]["ms"]
@@ -666,7 +691,15 @@ def bench_adjoint(args):
                           ms_per_step=round(out["mixed"]["ms"], 3),
                           grad_rel_err_vs_fp64=gerr, loss_rel_err_vs_fp64=lerr,
                           passes_err_gate=gerr <= ERR_GATE),
+            "int8x": dict(solves_per_s=round(nE / (out["int8x"]["ms"] / 1e3), 3),
+                          ms_per_step=round(out["int8x"]["ms"], 3),
+                          grad_rel_err_vs_fp64=gerr8, loss_rel_err_vs_fp64=lerr8,
+                          passes_err_gate=gerr8 <= ERR_GATE,
+                          peak_hbm_gb=round(out["int8x"]["peak_gb"], 2)),
+            "plain_fp64_batch1": plain,
         },
+        "speedup_vs_plain_fp64": {m: round(nE / (out[m]["ms"] / 1e3) / plain["solves_per_s"], 2)
+                                  for m in ("fp64", "mixed", "int8x")},
         "roofline": dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_TFLOPS["gemm_f64"],
                          unit="TFLOP/s", frac=round(achieved / PEAK_TFLOPS["gemm_f64"], 4),
                          traffic=None, kernel="gemm_kernel<double>", launches=kcnt,

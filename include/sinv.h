@@ -319,6 +319,12 @@ int sinv_zgemm_int8x_batched(int M, int N, int K, int opA, int opB, sinv_c128 al
                              const void* shift, int batch, int slices,
                              void* work, size_t work_bytes, void* stream);
 
+/* Product kernel of the int8x GEMM (no counterpart in the reference): 0 =
+ * two LDS buffers, one workgroup per CU; 1 = one LDS buffer, two workgroups
+ * per CU (default); -1 restores the default.  Both give identical bits (the
+ * digit sums are exact and the fold order is the same).  Process-wide. */
+int sinv_int8x_force_variant(int variant);
+
 /* ------------------------------------------------------------------------
  * Dense <-> block-tridiagonal conversion on the device.
  * Replaces matrices_utils.py:115-151 (convertDenseToBlkTridiag) and

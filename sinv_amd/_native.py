@@ -139,6 +139,8 @@ def lib():
     L.sinv_zgemm_int8x_batched.argtypes = [_i, _i, _i, _i, _i, C128, _vp, _i, _ll, _vp, _i, _ll,
                                            C128, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _i, _vp,
                                            ctypes.c_size_t, _vp]
+    L.sinv_int8x_force_variant.restype = _i
+    L.sinv_int8x_force_variant.argtypes = [_i]
     L.sinv_prof_begin.restype = _i
     L.sinv_prof_begin.argtypes = [_i]
     L.sinv_prof_end.restype = _i

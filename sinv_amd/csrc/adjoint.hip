@@ -31,7 +31,9 @@ namespace {
 bool adj_desc_ok(const sinv_rgf_adjoint_desc* d) {
   if (d == nullptr || d->nblocks <= 0 || d->batch <= 0) return false;
   if (d->bsizes == nullptr && d->blocksize <= 0) return false;
-  if (d->precision != SINV_PREC_FP64 && d->precision != SINV_PREC_MIXED) return false;
+  if (d->precision != SINV_PREC_FP64 && d->precision != SINV_PREC_MIXED &&
+      d->precision != SINV_PREC_INT8X)
+    return false;
   if (d->diag == nullptr || d->gbar_diag == nullptr || d->abar_diag == nullptr) return false;
   if (d->nblocks > 1 && (d->upper == nullptr || d->lower == nullptr || d->gbar_upper == nullptr ||
                          d->gbar_lower == nullptr || d->abar_upper == nullptr ||
@@ -47,12 +49,13 @@ struct AdjPlan {
   int k = 1, nseg = 1;
   long long blocks = 0;  // per instance
   size_t inv_elems = 0;
+  size_t i8x_bytes = 0;  // digit planes of the sliced-int8 GEMMs (SINV_PREC_INT8X)
   size_t bytes = 0;
 };
 
 constexpr int kTemps = 12;
 
-AdjPlan adj_plan(const BlockLayout& L, int batch, int every) {
+AdjPlan adj_plan(const BlockLayout& L, int batch, int every, int precision) {
   AdjPlan P;
   const int N = L.N;
   int k = every;
@@ -67,7 +70,17 @@ AdjPlan adj_plan(const BlockLayout& L, int batch, int every) {
     if (e > inv) inv = e;
   }
   P.inv_elems = inv;
-  P.bytes = ((size_t)P.blocks * L.bmax * L.bmax * batch + P.inv_elems) * sizeof(cplx) + 512;
+  if (precision == SINV_PREC_INT8X) {
+    // largest product: bmax^3 with both operands distinct per instance
+    GemmArgs g{};
+    g.M = g.N = g.K = L.bmax;
+    g.sA_out = g.sB_out = 1;
+    g.inner = 1;
+    g.outer = batch;
+    P.i8x_bytes = gemm_i8x_workspace_bytes(g, kI8xDefaultSlices);
+  }
+  P.bytes = ((size_t)P.blocks * L.bmax * L.bmax * batch + P.inv_elems) * sizeof(cplx) +
+            P.i8x_bytes + 512;
   return P;
 }
 
@@ -77,7 +90,7 @@ extern "C" size_t sinv_rgf_adjoint_workspace_bytes(const sinv_rgf_adjoint_desc* 
   if (!adj_desc_ok(d)) return 0;
   const BlockLayout L = make_block_layout(d->nblocks, d->blocksize, d->bsizes);
   if (!L.ok) return 0;
-  return adj_plan(L, d->batch, d->checkpoint_every).bytes;
+  return adj_plan(L, d->batch, d->checkpoint_every, d->precision).bytes;
 }
 
 extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* work,
@@ -92,17 +105,19 @@ extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* wo
     return SINV_E_INVAL;
   }
   const int N = Ly.N, nb = d->batch;
-  const AdjPlan P = adj_plan(Ly, nb, d->checkpoint_every);
+  const AdjPlan P = adj_plan(Ly, nb, d->checkpoint_every, d->precision);
   if (work == nullptr || work_bytes < P.bytes) {
     set_error("sinv_rgf_adjoint_batched: workspace too small (%zu < %zu)", work_bytes, P.bytes);
     return SINV_E_INVAL;
   }
   hipStream_t s = (hipStream_t)stream;
-  const Prec prec = d->precision == SINV_PREC_FP64 ? Prec::F64 : Prec::F32;
+  const Prec prec = d->precision == SINV_PREC_MIXED ? Prec::F32 : Prec::F64;
+  const bool i8x = d->precision == SINV_PREC_INT8X;
   const long long b2 = (long long)Ly.bmax * Ly.bmax;  // slot size
   cplx* base = (cplx*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
   const long long WI = P.blocks * b2;  // instance stride in the workspace
   cplx* winv = base + WI * nb;
+  void* wi8x = winv + P.inv_elems;
   auto wblk = [&](long long k) { return base + k * b2; };
   // temporaries
   cplx* Bd = wblk(0);
@@ -158,6 +173,7 @@ extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* wo
     g.beta = C0 ? one : zero;
     g.shift = nullptr;
     g.inner = 1; g.outer = nb;
+    if (i8x) return launch_gemm_i8x(g, kI8xDefaultSlices, wi8x, P.i8x_bytes, s);
     return launch_gemm(g, prec, s);
   };
   auto mm = [&](int M, int Nn, int K, Opd A, Opd B, cplx* C, long long cinst, int ldc, cplx alpha,

@@ -66,9 +66,12 @@ __host__ __device__ inline long long zindex(int mode, int z_in, int z_out, int i
 }
 
 // op(X) seen as R "rows" of K elements: element (r, k) at X + r*rs + k*ks,
-// imaginary part negated when conj.  Digits of distinct copy zi, part p,
-// digit t at data + ((zi*3 + p)*S + t)*Rp*Kp + r*Kp + k; scales at
-// scale + (zi*3 + p)*Rp + r (2^e, 0 for an all-zero row).
+// imaginary part negated when conj.  Digit t of part p of element (r, k) of
+// distinct copy zi is stored blocked by 64-wide k chunks, so that one chunk's
+// 64 x 64 digit tile is 4 KB of consecutive bytes:
+//   data + zi*3*S*Rp*Kp + (((p*nkc + k/64)*S + t)*Rp + r)*64 + k%64
+// (nkc = Kp/64); scales at scale + (zi*3 + p)*Rp + r (2^e, 0 for an all-zero
+// row).
 struct SliceArgs {
   const cplx* X;
   long long s_in, s_out;
@@ -122,6 +125,12 @@ __device__ __forceinline__ void digits(double y, int q[S]) {
   }
 }
 
+// byte offset of digit t, part p, row r, k chunk kc (k%64 added by the caller)
+__host__ __device__ __forceinline__ size_t dig_off(int S, int nkc, int Rp, int p, int kc, int t,
+                                                   int r) {
+  return ((((size_t)p * nkc + kc) * S + t) * Rp + r) * 64;
+}
+
 __device__ __forceinline__ unsigned pack4(int a, int b, int c, int d) {
   return (unsigned)(a & 0xff) | ((unsigned)(b & 0xff) << 8) | ((unsigned)(c & 0xff) << 16) |
          ((unsigned)(d & 0xff) << 24);
@@ -135,15 +144,16 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(SliceArgs a) {
   const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
   const long long zi = blockIdx.y;
   const cplx* X = slice_src(a, zi);
-  const size_t plane = (size_t)a.Rp * a.Kp;
-  int8_t* base = a.data + (size_t)zi * 3 * S * plane + (size_t)r * a.Kp;
+  const int nkc = a.Kp / 64;
+  int8_t* base = a.data + (size_t)zi * 3 * S * a.Rp * a.Kp;
   if (r >= a.R) {
     for (int k = 4 * lane; k < a.Kp; k += 256)
 #pragma unroll
       for (int p = 0; p < 3; ++p)
 #pragma unroll
         for (int t = 0; t < S; ++t)
-          *reinterpret_cast<unsigned*>(base + (size_t)(p * S + t) * plane + k) = 0u;
+          *reinterpret_cast<unsigned*>(base + dig_off(S, nkc, a.Rp, p, k >> 6, t, r) + (k & 63)) =
+              0u;
     if (lane < 3) a.scale[(zi * 3 + lane) * a.Rp + r] = 0.0;
     return;
   }
@@ -186,7 +196,8 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(SliceArgs a) {
     for (int p = 0; p < 3; ++p)
 #pragma unroll
       for (int t = 0; t < S; ++t)
-        *reinterpret_cast<unsigned*>(base + (size_t)(p * S + t) * plane + k0) =
+        *reinterpret_cast<unsigned*>(base + dig_off(S, nkc, a.Rp, p, k0 >> 6, t, r) +
+                                     (k0 & 63)) =
             pack4(q[p][0][t], q[p][1][t], q[p][2][t], q[p][3][t]);
   }
 }
@@ -240,7 +251,6 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(SliceArgs a, const doub
   const int kc = blockIdx.y;
   const long long zi = blockIdx.z;
   const cplx* X = slice_src(a, zi);
-  const size_t plane = (size_t)a.Rp * a.Kp;
   double inv[3];
 #pragma unroll
   for (int p = 0; p < 3; ++p) {
@@ -271,7 +281,7 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(SliceArgs a, const doub
       for (int t = 0; t < S; ++t)
         words[p][t][j4] = pack4(q[p][0][t], q[p][1][t], q[p][2][t], q[p][3][t]);
   }
-  int8_t* base = a.data + (size_t)zi * 3 * S * plane + (size_t)r * a.Kp + kg * 16;
+  int8_t* base = a.data + (size_t)zi * 3 * S * a.Rp * a.Kp + w * 16;
 #pragma unroll
   for (int p = 0; p < 3; ++p)
 #pragma unroll
@@ -281,7 +291,7 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(SliceArgs a, const doub
       v4.y = (int)words[p][t][1];
       v4.z = (int)words[p][t][2];
       v4.w = (int)words[p][t][3];
-      *reinterpret_cast<i4v*>(base + (size_t)(p * S + t) * plane) = v4;
+      *reinterpret_cast<i4v*>(base + dig_off(S, nkc, a.Rp, p, kc, t, r)) = v4;
     }
 }
 
@@ -319,21 +329,18 @@ __global__ __launch_bounds__(256) void gemm_i8x_kernel(GemmArgs g, ProdArgs o, i
 
   const long long za = zindex(o.a_mode, z_in, z_out, g.inner);
   const long long zb = zindex(o.b_mode, z_in, z_out, g.inner);
-  const size_t a_plane = (size_t)o.a_Rp * Kp, b_plane = (size_t)o.b_Rp * Kp;
   // this thread's 16-byte piece of every digit tile: row tid / 4, bytes 16 (tid % 4)
   const int lr = tid >> 2, lc = (tid & 3) * 16;
-  const int8_t* a_src = o.a_data + (size_t)za * 3 * S * a_plane + (size_t)(m0 + lr) * Kp + lc;
-  const int8_t* b_src = o.b_data + (size_t)zb * 3 * S * b_plane + (size_t)(n0 + lr) * Kp + lc;
+  const int8_t* a_src = o.a_data + (size_t)za * 3 * S * o.a_Rp * Kp + (size_t)(m0 + lr) * 64 + lc;
+  const int8_t* b_src = o.b_data + (size_t)zb * 3 * S * o.b_Rp * Kp + (size_t)(n0 + lr) * 64 + lc;
 
   i4v ra[S], rb[S];
   auto load = [&](int c) {
     const int p = c / nk, kc = c - p * nk;
-    const size_t ka = (size_t)p * S * a_plane + (size_t)kc * kChunk;
-    const size_t kb = (size_t)p * S * b_plane + (size_t)kc * kChunk;
 #pragma unroll
     for (int d = 0; d < S; ++d) {
-      ra[d] = *reinterpret_cast<const i4v*>(a_src + ka + d * a_plane);
-      rb[d] = *reinterpret_cast<const i4v*>(b_src + kb + d * b_plane);
+      ra[d] = *reinterpret_cast<const i4v*>(a_src + dig_off(S, nk, o.a_Rp, p, kc, d, 0));
+      rb[d] = *reinterpret_cast<const i4v*>(b_src + dig_off(S, nk, o.b_Rp, p, kc, d, 0));
     }
   };
   auto store = [&](int buf) {
@@ -452,6 +459,144 @@ __global__ __launch_bounds__(256) void gemm_i8x_kernel(GemmArgs g, ProdArgs o, i
       }
 }
 
+// Same product with a single LDS buffer (S x 64 x 80 bytes per operand, so
+// two workgroups share a CU) and registers capped for two waves per SIMD:
+// each chunk is loaded, stored to LDS between two barriers and then
+// multiplied; the other workgroup's MFMAs cover this one's load latency.
+template <int S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void gemm_i8x_o2_kernel(GemmArgs g, ProdArgs o, int tiles_m, int tiles_n) {
+  __shared__ __attribute__((aligned(16))) int8_t lds[2][S][kTile][kRowB];
+
+  int bid = blockIdx.x;
+  const int total = gridDim.x;
+  if ((total & 7) == 0) bid = (bid & 7) * (total >> 3) + (bid >> 3);
+  const int tiles = tiles_m * tiles_n;
+  const int z = bid / tiles;
+  const int t = bid - z * tiles;
+  const int z_out = z / g.inner, z_in = z - z_out * g.inner;
+  const int m0 = (t / tiles_n) * kTile, n0 = (t % tiles_n) * kTile;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int Kp = o.Kp, nk = Kp / kChunk;
+
+  const long long za = zindex(o.a_mode, z_in, z_out, g.inner);
+  const long long zb = zindex(o.b_mode, z_in, z_out, g.inner);
+  const int lr = tid >> 2, lc = (tid & 3) * 16;
+  const int8_t* a_src = o.a_data + (size_t)za * 3 * S * o.a_Rp * Kp + (size_t)(m0 + lr) * 64 + lc;
+  const int8_t* b_src = o.b_data + (size_t)zb * 3 * S * o.b_Rp * Kp + (size_t)(n0 + lr) * 64 + lc;
+  const int fr = lane & 15, fk = (lane >> 4) * 16;
+
+  double cr[2][2][4], ci[2][2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) cr[i][j][v] = ci[i][j][v] = 0.0;
+
+#pragma unroll 1
+  for (int p = 0; p < 3; ++p) {
+    i4v acc[2][2][S];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int d = 0; d < S; ++d) acc[i][j][d] = i4v{0, 0, 0, 0};
+#pragma unroll 1
+    for (int kc = 0; kc < nk; ++kc) {
+      {
+        i4v ra[S], rb[S];
+#pragma unroll
+        for (int d = 0; d < S; ++d) {
+          ra[d] = *reinterpret_cast<const i4v*>(a_src + dig_off(S, nk, o.a_Rp, p, kc, d, 0));
+          rb[d] = *reinterpret_cast<const i4v*>(b_src + dig_off(S, nk, o.b_Rp, p, kc, d, 0));
+        }
+        __syncthreads();  // every wave is done with the previous chunk
+#pragma unroll
+        for (int d = 0; d < S; ++d) {
+          *reinterpret_cast<i4v*>(&lds[0][d][lr][lc]) = ra[d];
+          *reinterpret_cast<i4v*>(&lds[1][d][lr][lc]) = rb[d];
+        }
+        __syncthreads();
+      }
+      // one column tile at a time (its S digit fragments in registers); the
+      // A fragments are read once per column tile
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        i4v bf[S];
+#pragma unroll
+        for (int u = 0; u < S; ++u)
+          bf[u] = *reinterpret_cast<const i4v*>(&lds[1][u][wn * 32 + ct * 16 + fr][fk]);
+#pragma unroll
+        for (int ta = 0; ta < S; ++ta)
+#pragma unroll
+          for (int rt = 0; rt < 2; ++rt) {
+            const i4v af =
+                *reinterpret_cast<const i4v*>(&lds[0][ta][wm * 32 + rt * 16 + fr][fk]);
+#pragma unroll
+            for (int ub = 0; ta + ub < S; ++ub)
+              acc[rt][ct][ta + ub] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                  af, bf[ub], acc[rt][ct][ta + ub], 0, 0, 0);
+          }
+      }
+    }
+    const double* sa = o.a_scale + ((size_t)za * 3 + p) * o.a_Rp + m0 + wm * 32;
+    const double* sb = o.b_scale + ((size_t)zb * 3 + p) * o.b_Rp + n0 + wn * 32;
+    const double sr = p == 0 ? 1.0 : -1.0;
+    const double si = p == 2 ? 1.0 : -1.0;
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        const double scol = sb[ct * 16 + fr];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double val = 0.0;
+#pragma unroll
+          for (int d = S - 1; d >= 0; --d)
+            val += (double)acc[rt][ct][d][v] * __builtin_ldexp(1.0, -7 * (d + 2));
+          val *= sa[rt * 16 + (lane >> 4) * 4 + v] * scol;
+          if (p < 2) cr[rt][ct][v] += sr * val;
+          ci[rt][ct][v] += si * val;
+        }
+      }
+  }
+
+  cplx* C = g.C + z_in * g.sC_in + z_out * g.sC_out;
+  const cplx* C0 = g.C0 ? g.C0 + z_in * g.sC0_in + z_out * g.sC0_out : nullptr;
+  const cplx sh = g.shift ? g.shift[z_out] : cplx{0.0, 0.0};
+#pragma unroll
+  for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = m0 + wm * 32 + rt * 16 + (lane >> 4) * 4 + v;
+        const int col = n0 + wn * 32 + ct * 16 + fr;
+        if (row < g.M && col < g.N) {
+          const double tr = cr[rt][ct][v], ti = ci[rt][ct][v];
+          double re = g.alpha.re * tr - g.alpha.im * ti;
+          double im = g.alpha.re * ti + g.alpha.im * tr;
+          if (C0) {
+            const cplx c0 = C0[(long long)row * g.ldc0 + col];
+            re += g.beta.re * c0.re - g.beta.im * c0.im;
+            im += g.beta.re * c0.im + g.beta.im * c0.re;
+          }
+          if (row == col) {
+            re += sh.re;
+            im += sh.im;
+          }
+          C[(long long)row * g.ldc + col] = cplx{re, im};
+        }
+      }
+}
+
+int g_i8x_variant = -1;  // forced product kernel (-1: default)
+constexpr int kI8xDefaultVariant = 1;
+
 struct I8xPlan {
   int a_mode, b_mode;
   long long a_nz, b_nz;
@@ -531,7 +676,12 @@ hipError_t launch_i8x_s(const GemmArgs& g, const I8xPlan& p, uint8_t* base, hipS
   o.Kp = p.Kp;
   const int tm = p.Mp / kTile, tn = p.Np / kTile;
   const long long blocks = (long long)tm * tn * g.inner * g.outer;
-  hipLaunchKernelGGL(gemm_i8x_kernel<S>, dim3((unsigned)blocks), dim3(256), 0, s, g, o, tm, tn);
+  const int variant = g_i8x_variant >= 0 ? g_i8x_variant : kI8xDefaultVariant;
+  if (variant == 0)
+    hipLaunchKernelGGL(gemm_i8x_kernel<S>, dim3((unsigned)blocks), dim3(256), 0, s, g, o, tm, tn);
+  else
+    hipLaunchKernelGGL(gemm_i8x_o2_kernel<S>, dim3((unsigned)blocks), dim3(256), 0, s, g, o, tm,
+                       tn);
   return hipGetLastError();
 }
 
@@ -627,5 +777,14 @@ extern "C" int sinv_zgemm_int8x_batched(int M, int N, int K, int opA, int opB, s
     return SINV_E_INVAL;
   }
   SINV_HIP_TRY(launch_gemm_i8x(g, slices, work, work_bytes, (hipStream_t)stream));
+  return SINV_OK;
+}
+
+extern "C" int sinv_int8x_force_variant(int variant) {
+  if (variant < -1 || variant > 1) {
+    set_error("sinv_int8x_force_variant: variant must be -1, 0 or 1");
+    return SINV_E_INVAL;
+  }
+  sinv::g_i8x_variant = variant;
   return SINV_OK;
 }

@@ -163,3 +163,60 @@ def test_rgf_int8x_ragged_and_unsaved(built_lib):
     ref = rgf_ref.rgf_blocks(dd, uu, ll)
     assert np.linalg.norm(Gd[0].cpu().numpy() - ref[0]) <= 1e-12 * np.linalg.norm(ref[0])
     assert np.isnan(Gu.cpu().numpy()).all() and np.isnan(Gl.cpu().numpy()).all()
+
+
+def test_zgemm_int8x_product_variants_identical(built_lib):
+    """Both product kernels (two LDS buffers / one) give the same bits."""
+    import torch
+    from sinv_amd import core, _native as nat
+    rng = np.random.default_rng(21)
+    A = _rand(rng, (4, 200, 150))
+    B = _rand(rng, (1, 150, 130))
+    L = nat.lib()
+    outs = []
+    try:
+        for v in (0, 1):
+            nat.check(L.sinv_int8x_force_variant(v), "force_variant")
+            outs.append(core.zgemm_batched(torch.from_numpy(A), torch.from_numpy(B),
+                                           precision="int8x").cpu().numpy())
+    finally:
+        nat.check(L.sinv_int8x_force_variant(-1), "force_variant")
+    assert np.array_equal(outs[0], outs[1])
+    assert L.sinv_int8x_force_variant(2) == nat.SINV_E_INVAL
+
+
+def test_adjoint_int8x_against_dense_formula_and_fp64(built_lib):
+    """The adjoint sweep with int8x GEMMs: within 1e-10 of the dense formula on
+    a dominant case, and within 1e-8 of the FP64 adjoint on energies near the
+    band (eta = 1e-3), ragged blocks, checkpointed."""
+    from oracle import adjoint_ref
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    rng = np.random.default_rng(17)
+    N, b = 6, 80
+    d, u, l = configs.dominant_blocks(N, b, seed=3)
+    c = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
+    gd, gu, gl = c(N, b, b), c(N - 1, b, b), c(N - 1, b, b)
+    got = [x[0].cpu().numpy() for x in
+           core.rgf_adjoint_batched(d, u, l, gd, gu, gl, precision="int8x")]
+    ref = adjoint_ref.grad_dense(d, u, l, gd, gu, gl)
+    num = sum(np.linalg.norm(a - r) ** 2 for a, r in zip(got, ref))
+    den = sum(np.linalg.norm(r) ** 2 for r in ref)
+    assert np.sqrt(num / den) < 1e-10
+
+    bs = [72, 130, 96, 64, 110, 80, 72]
+    H = configs.hermitian_H_ragged(bs, seed=6)
+    E = configs.energies(6, seed=8) + 1e-3j
+    gdl = [(-1j / np.pi) * np.eye(x) for x in bs]
+    zu = [np.zeros((bs[i], bs[i + 1]), complex) for i in range(len(bs) - 1)]
+    zl = [np.zeros((bs[i + 1], bs[i]), complex) for i in range(len(bs) - 1)]
+    outs = {}
+    for prec in ("fp64", "int8x"):
+        outs[prec] = core.rgf_adjoint_ragged(*H, gdl, zu, zl, shift=E, dscale=-1.0, oscale=-1.0,
+                                             precision=prec, checkpoint_every=3)
+    num = den = 0.0
+    for xs, ys in zip(outs["int8x"], outs["fp64"]):
+        for a, r in zip(xs, ys):
+            num += float(((a - r).abs() ** 2).sum())
+            den += float((r.abs() ** 2).sum())
+    assert np.sqrt(num / den) < 1e-8, np.sqrt(num / den)
