@@ -52,7 +52,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--energies", type=int, default=256)
     ap.add_argument("--workload", default="energies",
-                    choices=["energies", "dominant", "large", "adjoint", "convert", "variants"])
+                    choices=["energies", "dominant", "large", "adjoint", "convert", "variants",
+                             "gemm"])
     ap.add_argument("--chunk", type=int, default=64, help="energies per adjoint launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the batch=1 FP64 sample")
@@ -132,6 +133,8 @@ def main():
         return bench_convert(args)
     if args.workload == "variants":
         return bench_variants(args)
+    if args.workload == "gemm":
+        return bench_gemm(args)
 
     # ------------------------------------------------------------ workload
     eta = 1e-3
@@ -710,6 +713,65 @@ def bench_adjoint(args):
     print(json.dumps(res), flush=True)
     return 0
 
+
+
+def bench_gemm(args):
+    """The headline's Schur-complement product shape on its own: 256 products
+    b x b x b (complex128), the left operand per energy and the right one
+    shared (stride 0), for b in {128, 256, 384}: the FP64 MFMA kernel (tuned
+    config) against the int8x emulation (digit split of both operands plus the
+    int8 product kernel, both inside the timed call), with the int8x error
+    against the FP64 result.  Event-timed over `steps` calls after `warmup`."""
+    import torch
+    from sinv_amd import core, _native
+    dev = core.device()
+    gen = torch.Generator(device=dev).manual_seed(7)
+    rows = []
+    for b in (128, 256, 384):
+        nb = 256
+        A = torch.complex(torch.randn(nb, b, b, dtype=torch.float64, device=dev, generator=gen),
+                          torch.randn(nb, b, b, dtype=torch.float64, device=dev, generator=gen))
+        H = torch.complex(torch.randn(1, b, b, dtype=torch.float64, device=dev, generator=gen),
+                          torch.randn(1, b, b, dtype=torch.float64, device=dev, generator=gen))
+        row = dict(b=b, batch=nb)
+        out = {}
+        for prec in ("fp64", "int8x"):
+            if prec == "fp64" and not args.no_tune:
+                from sinv_amd import autotune
+                autotune.tune(lambda: core.zgemm_batched(A, H, precision="fp64"), reps=2)
+            for _ in range(max(args.warmup, 1)):
+                out[prec] = core.zgemm_batched(A, H, precision=prec)
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(args.steps):
+                out[prec] = core.zgemm_batched(A, H, precision=prec)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / args.steps
+            with _native.LaunchProfiler() as prof:
+                core.zgemm_batched(A, H, precision=prec)
+                torch.cuda.synchronize()
+            kind = "gemm_f64" if prec == "fp64" else "gemm_i8x"
+            row[prec] = dict(ms=round(ms, 4), tflops=round(8 * b ** 3 * nb / (ms / 1e3) / 1e12, 2),
+                             profiled_call_ms=round(prof.stats[kind][0], 4))
+        ref = out["fp64"]
+        row["int8x"]["rel_err_vs_fp64"] = float((out["int8x"] - ref).norm() / ref.norm())
+        row["int8x_over_fp64_time"] = round(row["int8x"]["ms"] / row["fp64"]["ms"], 3)
+        log(f"[bench] gemm b={b}: {row}")
+        rows.append(row)
+        del A, H, out, ref
+    res = {
+        "metric": "complex GEMM time, FP64 MFMA vs int8x emulation (headline Schur product shapes)",
+        "value": rows[-1]["int8x"]["tflops"], "unit": "TFLOP/s (8MNK, int8x, b=384)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+        "dtype": "complex128", "data": "synthetic",
+        "config": dict(workload="zgemm_headline_shapes", products=256, right_operand="shared"),
+        "shapes": rows,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
 
 
 def bench_convert(args):

@@ -10,7 +10,9 @@
 // 7-bit digits by exact double arithmetic:
 //     y <- 128 y;  q_t = trunc(y);  y <- y - q_t     (t = 0 .. S-1)
 // so x = 2^e (sum_t q_t 2^(-7(t+1)) + rem), |rem| < 2^(-7S), |q_t| <= 127.
-// The digit products q_t(A) q_u(B) summed over k are exact in int32 on
+// (The digits are taken as the base-128 digits of floor(|x / 2^e| 128^S) with
+// x's sign, which the sequential form yields.)  The digit products
+// q_t(A) q_u(B) summed over k are exact in int32 on
 // v_mfma_i32_16x16x64_i8; pairs with the same t + u share one accumulator, and
 // pairs with t + u > S - 1 are dropped.  After the k loop each accumulator is
 // converted to double and scaled by 2^(-7(t+u+2)) 2^(eA_r + eB_c).  Per term
@@ -23,10 +25,11 @@
 // product uses the 3M form as the FP64 kernel does: T1 = Ar Br, T2 = Ai Bi,
 // T3 = (Ar + Ai)(Br + Bi), C = (T1 - T2) + i (T3 - T1 - T2).
 //
-// Data flow per launch: two slicing kernels write the digits of op(A) and
-// op(B) (int8, row-major along k, k padded to a multiple of 64 and rows to a
-// multiple of 64 with zeros) and their scales into the caller's workspace;
-// an operand with a zero batch stride is sliced once.  The product kernel
+// Data flow per launch: per operand, a row-maximum pass and a digit pass over
+// 64 x 64 pieces write the digits of op(A) and op(B) (int8, blocked by 64-wide
+// k chunks, k and rows padded to multiples of 64 with zeros) and their scales
+// into the caller's workspace; an operand with a zero batch stride is sliced
+// once.  The product kernel
 // stages 64 x 64 digit tiles of all S digits of op(A) and op(B) through two
 // LDS buffers (one barrier per 64-wide k chunk), runs the three real products
 // one after another over k, and folds each into double accumulators.
@@ -112,16 +115,31 @@ __device__ __forceinline__ void scale_of(double mx, double& sc, double& inv) {
   }
 }
 
-// S digits of y in (-1, 1): exact in double (scaling by 128 and removing the
-// integer part lose nothing), each digit in [-127, 127].
+// S digits of y in (-1, 1), most significant first: the base-128 digits of
+// floor(|y| 128^S) carrying y's sign, which is what the sequential form
+//     y <- 128 y;  q_t = trunc(y);  y <- y - q_t
+// produces (trunc keeps every remainder on y's side of zero).  |y| 128^S is
+// exact in double (a power-of-two scaling) and below 2^(7S); it is split
+// exactly into a high part of 4 digits (28 bits) and a low part of S - 4
+// digits, each converted to a 32-bit integer once.
 template <int S>
 __device__ __forceinline__ void digits(double y, int q[S]) {
+  static_assert(S >= 5 && S <= 8, "4 high digits plus 1..4 low digits");
+  constexpr int LB = 7 * (S - 4);
+  const double a = __builtin_ldexp(fabs(y), 7 * S);
+  const double hd = trunc(__builtin_ldexp(a, -LB));
+  const unsigned hi = (unsigned)hd;
+  const unsigned lo = (unsigned)(a - __builtin_ldexp(hd, LB));  // truncates: drops the remainder
+  const int neg = y < 0.0;
 #pragma unroll
-  for (int t = 0; t < S; ++t) {
-    y *= 128.0;
-    const double d = trunc(y);
-    q[t] = (int)d;
-    y -= d;
+  for (int t = 0; t < 4; ++t) {
+    const int d = (int)((hi >> (7 * (3 - t))) & 127u);
+    q[t] = neg ? -d : d;
+  }
+#pragma unroll
+  for (int t = 4; t < S; ++t) {
+    const int d = (int)((lo >> (7 * (S - 1 - t))) & 127u);
+    q[t] = neg ? -d : d;
   }
 }
 
@@ -136,118 +154,76 @@ __device__ __forceinline__ unsigned pack4(int a, int b, int c, int d) {
          ((unsigned)(d & 0xff) << 24);
 }
 
-// k-contiguous operand (ks == 1): one wave per row, lane l takes k in
-// [4l + 256j, 4l + 256j + 4).  grid (Rp / 4, distinct copies), 256 threads.
-template <int S>
-__global__ __launch_bounds__(256) void slice_rows_kernel(SliceArgs a) {
-  const int lane = threadIdx.x & 63;
-  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const long long zi = blockIdx.y;
-  const cplx* X = slice_src(a, zi);
-  const int nkc = a.Kp / 64;
-  int8_t* base = a.data + (size_t)zi * 3 * S * a.Rp * a.Kp;
-  if (r >= a.R) {
-    for (int k = 4 * lane; k < a.Kp; k += 256)
-#pragma unroll
-      for (int p = 0; p < 3; ++p)
-#pragma unroll
-        for (int t = 0; t < S; ++t)
-          *reinterpret_cast<unsigned*>(base + dig_off(S, nkc, a.Rp, p, k >> 6, t, r) + (k & 63)) =
-              0u;
-    if (lane < 3) a.scale[(zi * 3 + lane) * a.Rp + r] = 0.0;
-    return;
-  }
-  const cplx* row = X + (long long)r * a.rs;
-  double mx[3] = {0.0, 0.0, 0.0};
-  for (int k0 = 4 * lane; k0 < a.K; k0 += 256) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int k = k0 + q;
-      if (k < a.K) {
-        double v[3];
-        parts_of(row[k], a.conj, v);
-#pragma unroll
-        for (int p = 0; p < 3; ++p) mx[p] = fmax(mx[p], fabs(v[p]));
-      }
-    }
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-    for (int p = 0; p < 3; ++p) mx[p] = fmax(mx[p], __shfl_xor(mx[p], off));
-  double inv[3];
-#pragma unroll
-  for (int p = 0; p < 3; ++p) {
-    double sc;
-    scale_of(mx[p], sc, inv[p]);
-    if (lane == 0) a.scale[(zi * 3 + p) * a.Rp + r] = sc;
-  }
-  for (int k0 = 4 * lane; k0 < a.Kp; k0 += 256) {
-    int q[3][4][S];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = k0 + j;
-      double v[3] = {0.0, 0.0, 0.0};
-      if (k < a.K) parts_of(row[k], a.conj, v);
-#pragma unroll
-      for (int p = 0; p < 3; ++p) digits<S>(v[p] * inv[p], q[p][j]);
-    }
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-      for (int t = 0; t < S; ++t)
-        *reinterpret_cast<unsigned*>(base + dig_off(S, nkc, a.Rp, p, k0 >> 6, t, r) +
-                                     (k0 & 63)) =
-            pack4(q[p][0][t], q[p][1][t], q[p][2][t], q[p][3][t]);
-  }
-}
-
-// row-contiguous operand (rs == 1), two passes over 64 x 64 pieces (rows
-// 64 blockIdx.x + lane, k in 64 blockIdx.y + [16 w, 16 w + 16) for wave w),
-// grid (Rp / 64, Kp / 64, distinct copies), 256 threads:
-//  1. colmax: the piece's row maxima per part -> pmax[(zi*nkc + kc)*3 + p][r]
-//  2. slice: each row's scale from its nkc partial maxima, then the piece's
-//     digits (16 consecutive k per thread, one 16-byte store per plane).
+// Two passes over 64 x 64 pieces of op(X) (rows 64 blockIdx.x + [0, 64),
+// k in 64 blockIdx.y + [0, 64)), grid (Rp / 64, Kp / 64, distinct copies),
+// 256 threads; thread (rl, kg) takes row 64 blockIdx.x + rl and the 16
+// consecutive k of group kg:
+//   KC = false (row-contiguous storage, rs == 1): rl = lane, kg = wave, so a
+//     wave's loads are runs along the rows;
+//   KC = true (k-contiguous storage, ks == 1): rl = tid / 4, kg = tid % 4, so
+//     four threads read one row's 64 consecutive elements.
+// Either way a thread's digits for one plane are 16 consecutive bytes of the
+// blocked layout, and a wave's stores cover whole 64-byte rows.
+//  1. piece_max: the piece's row maxima per part -> pmax[(zi*nkc + kc)*3 + p][r]
+//  2. slice_piece: each row's scale from its nkc partial maxima, then the
+//     piece's digits.
 __device__ __forceinline__ double* pmax_at(double* pmax, long long zi, int nkc, int kc, int p,
                                            int Rp) {
   return pmax + ((zi * nkc + kc) * 3 + p) * (long long)Rp;
 }
 
-__global__ __launch_bounds__(256) void colmax_kernel(SliceArgs a, double* pmax, int nkc) {
+template <bool KC>
+__device__ __forceinline__ void piece_coords(int& rl, int& kg) {
+  const int tid = threadIdx.x;
+  rl = KC ? tid >> 2 : tid & 63;
+  kg = KC ? tid & 3 : tid >> 6;
+}
+
+template <bool KC>
+__global__ __launch_bounds__(256) void piece_max_kernel(SliceArgs a, double* pmax, int nkc) {
   __shared__ double part[4][3][64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int r = blockIdx.x * 64 + lane;
+  int rl, kg;
+  piece_coords<KC>(rl, kg);
+  const int r = blockIdx.x * 64 + rl;
   const int kc = blockIdx.y;
   const long long zi = blockIdx.z;
   const cplx* X = slice_src(a, zi);
+  // a maximum does not care which k a thread sees: with k-contiguous storage
+  // the row's four threads interleave (k = 4 j + kg), so each load instruction
+  // reads 64 consecutive bytes of every row it touches
   double mx[3] = {0.0, 0.0, 0.0};
   if (r < a.R) {
-#pragma unroll 4
-    for (int j = 0; j < 16; ++j) {
-      const int k = kc * 64 + w * 16 + j;
-      if (k < a.K) {
-        double v[3];
-        parts_of(X[(long long)r + (long long)k * a.ks], a.conj, v);
+    cplx x[16];
 #pragma unroll
-        for (int p = 0; p < 3; ++p) mx[p] = fmax(mx[p], fabs(v[p]));
-      }
+    for (int j = 0; j < 16; ++j) {
+      const int k = kc * 64 + (KC ? 4 * j + kg : kg * 16 + j);
+      x[j] = k < a.K ? X[(long long)r * a.rs + (long long)k * a.ks] : cplx{0.0, 0.0};
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      double v[3];
+      parts_of(x[j], a.conj, v);
+#pragma unroll
+      for (int p = 0; p < 3; ++p) mx[p] = fmax(mx[p], fabs(v[p]));
     }
   }
 #pragma unroll
-  for (int p = 0; p < 3; ++p) part[w][p][lane] = mx[p];
+  for (int p = 0; p < 3; ++p) part[kg][p][rl] = mx[p];
   __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (w < 3) {
     const double m = fmax(fmax(part[0][w][lane], part[1][w][lane]),
                           fmax(part[2][w][lane], part[3][w][lane]));
-    pmax_at(pmax, zi, nkc, kc, w, a.Rp)[r] = m;
+    pmax_at(pmax, zi, nkc, kc, w, a.Rp)[blockIdx.x * 64 + lane] = m;
   }
 }
 
-template <int S>
-__global__ __launch_bounds__(256) void slice_cols_kernel(SliceArgs a, const double* pmax,
-                                                         int nkc) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int r = blockIdx.x * 64 + lane;
+template <int S, bool KC>
+__global__ __launch_bounds__(256) void slice_piece_kernel(SliceArgs a, const double* pmax,
+                                                          int nkc) {
+  int rl, kg_in;
+  piece_coords<KC>(rl, kg_in);
+  const int r = blockIdx.x * 64 + rl;
   const int kc = blockIdx.y;
   const long long zi = blockIdx.z;
   const cplx* X = slice_src(a, zi);
@@ -259,40 +235,42 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(SliceArgs a, const doub
       m = fmax(m, pmax_at(const_cast<double*>(pmax), zi, nkc, c, p, a.Rp)[r]);
     double sc;
     scale_of(m, sc, inv[p]);
-    if (kc == 0 && w == 0) a.scale[(zi * 3 + p) * a.Rp + r] = sc;
+    if (kc == 0 && kg_in == 0) a.scale[(zi * 3 + p) * a.Rp + r] = sc;
   }
   const bool live = r < a.R;
-  const int kg = kc * 4 + w;  // 16-wide k group
-  unsigned words[3][S][4];
+  const int kg = kc * 4 + kg_in;  // 16-wide k group
+  cplx x[16];
 #pragma unroll
-  for (int j4 = 0; j4 < 4; ++j4) {
-    int q[3][4][S];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = kg * 16 + j4 * 4 + j;
-      double v[3] = {0.0, 0.0, 0.0};
-      if (live && k < a.K) parts_of(X[(long long)r + (long long)k * a.ks], a.conj, v);
-#pragma unroll
-      for (int p = 0; p < 3; ++p) digits<S>(v[p] * inv[p], q[p][j]);
-    }
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-      for (int t = 0; t < S; ++t)
-        words[p][t][j4] = pack4(q[p][0][t], q[p][1][t], q[p][2][t], q[p][3][t]);
+  for (int j = 0; j < 16; ++j) {
+    const int k = kg * 16 + j;
+    x[j] = (live && k < a.K) ? X[(long long)r * a.rs + (long long)k * a.ks] : cplx{0.0, 0.0};
   }
-  int8_t* base = a.data + (size_t)zi * 3 * S * a.Rp * a.Kp + w * 16;
+  int8_t* base = a.data + (size_t)zi * 3 * S * a.Rp * a.Kp + kg_in * 16;
 #pragma unroll
-  for (int p = 0; p < 3; ++p)
+  for (int p = 0; p < 3; ++p) {
+    unsigned words[S][4];
+#pragma unroll
+    for (int j4 = 0; j4 < 4; ++j4) {
+      int q[4][S];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        double v[3];
+        parts_of(x[j4 * 4 + j], a.conj, v);
+        digits<S>(v[p] * inv[p], q[j]);
+      }
+#pragma unroll
+      for (int t = 0; t < S; ++t) words[t][j4] = pack4(q[0][t], q[1][t], q[2][t], q[3][t]);
+    }
 #pragma unroll
     for (int t = 0; t < S; ++t) {
       i4v v4;
-      v4.x = (int)words[p][t][0];
-      v4.y = (int)words[p][t][1];
-      v4.z = (int)words[p][t][2];
-      v4.w = (int)words[p][t][3];
+      v4.x = (int)words[t][0];
+      v4.y = (int)words[t][1];
+      v4.z = (int)words[t][2];
+      v4.w = (int)words[t][3];
       *reinterpret_cast<i4v*>(base + dig_off(S, nkc, a.Rp, p, kc, t, r)) = v4;
     }
+  }
 }
 
 struct ProdArgs {
@@ -656,16 +634,17 @@ hipError_t launch_i8x_s(const GemmArgs& g, const I8xPlan& p, uint8_t* base, hipS
   double* pmax = reinterpret_cast<double*>(base + p.pmax);
   for (const SliceArgs* a : {&sa, &sb}) {
     const unsigned nz = (unsigned)(a == &sa ? p.a_nz : p.b_nz);
+    // the two operands' passes run one after the other on the stream, so they
+    // share the partial-maximum buffer
+    const dim3 grid((unsigned)(a->Rp / 64), (unsigned)nkc, nz);
     if (a->ks == 1) {
-      hipLaunchKernelGGL(slice_rows_kernel<S>, dim3((unsigned)(a->Rp / 4), nz), dim3(256), 0, s,
-                         *a);
-    } else {
-      // the two operands' column passes run one after the other on the stream,
-      // so they share the partial-maximum buffer
-      const dim3 grid((unsigned)(a->Rp / 64), (unsigned)nkc, nz);
-      hipLaunchKernelGGL(colmax_kernel, grid, dim3(256), 0, s, *a, pmax, nkc);
+      hipLaunchKernelGGL(piece_max_kernel<true>, grid, dim3(256), 0, s, *a, pmax, nkc);
       SINV_HIP_RET(hipGetLastError());
-      hipLaunchKernelGGL(slice_cols_kernel<S>, grid, dim3(256), 0, s, *a, pmax, nkc);
+      hipLaunchKernelGGL((slice_piece_kernel<S, true>), grid, dim3(256), 0, s, *a, pmax, nkc);
+    } else {
+      hipLaunchKernelGGL(piece_max_kernel<false>, grid, dim3(256), 0, s, *a, pmax, nkc);
+      SINV_HIP_RET(hipGetLastError());
+      hipLaunchKernelGGL((slice_piece_kernel<S, false>), grid, dim3(256), 0, s, *a, pmax, nkc);
     }
     SINV_HIP_RET(hipGetLastError());
   }
