@@ -121,3 +121,34 @@ def test_converters_reject_bad_arguments_on_the_host(built_lib):
         == nat.SINV_E_INVAL  # upper/lower missing for 3 blocks
     assert built_lib.sinv_scatter_blocks(buf, 0, 1, buf, buf, 2, 8, 1, buf, 8, 256, None) \
         == nat.SINV_E_INVAL  # lda smaller than the matrix
+
+
+def test_int8x_host_validation(built_lib):
+    """int8x entry points reject bad arguments on the host (no GPU call), and the
+    RGF and adjoint workspaces grow by the digit planes in int8x mode."""
+    from sinv_amd import _native as nat
+    L = built_lib
+    assert L.sinv_zgemm_int8x_workspace_bytes(64, 64, 64, 1, 0, 2, 4) == 0
+    assert L.sinv_zgemm_int8x_workspace_bytes(64, 64, 64, 1, 0, 2, 8) == 0
+    ws7 = L.sinv_zgemm_int8x_workspace_bytes(100, 70, 130, 1, 0, 3, 7)
+    # digit planes: 3 copies of op(A) (128 x 192 padded) and 1 of op(B) (128 x 192), 3 parts x 7
+    assert ws7 >= (3 + 1) * 3 * 7 * 128 * 192
+    assert L.sinv_zgemm_int8x_workspace_bytes(100, 70, 130, 1, 0, 3, 5) < ws7
+    one = nat.C128.of(1)
+    rc = L.sinv_zgemm_int8x_batched(4, 4, 4, 0, 0, one, None, 4, 0, None, 4, 0, nat.C128.of(0),
+                                    None, 4, 0, None, 4, 0, None, 1, 4, None, 0, None)
+    assert rc == nat.SINV_E_INVAL
+    rc = L.sinv_zgemm_int8x_batched(4, 4, 20000, 0, 0, one, ctypes.c_void_p(16), 20000, 0,
+                                    ctypes.c_void_p(16), 4, 0, nat.C128.of(0), None, 4, 0,
+                                    ctypes.c_void_p(16), 4, 0, None, 1, 7, None, 0, None)
+    assert rc == nat.SINV_E_INVAL and b"exceeds" in L.sinv_last_error()
+    assert L.sinv_int8x_force_variant(2) == nat.SINV_E_INVAL
+    assert L.sinv_int8x_force_variant(-1) == nat.SINV_OK
+    d = nat.RgfDesc(nblocks=8, blocksize=96, batch=4, precision=0, diag=1, upper=1, lower=1,
+                    g_diag=1, g_upper=1, g_lower=1, save_off_diag=1)
+    w64 = L.sinv_rgf_workspace_bytes(ctypes.byref(d))
+    d.precision = 2
+    w8 = L.sinv_rgf_workspace_bytes(ctypes.byref(d))
+    assert w64 > 0 and w8 >= w64 + 2 * 4 * 3 * 7 * 128 * 128
+    d.precision = 3
+    assert L.sinv_rgf_workspace_bytes(ctypes.byref(d)) == 0
