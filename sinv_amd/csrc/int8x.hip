@@ -29,10 +29,11 @@
 // 64 x 64 pieces write the digits of op(A) and op(B) (int8, blocked by 64-wide
 // k chunks, k and rows padded to multiples of 64 with zeros) and their scales
 // into the caller's workspace; an operand with a zero batch stride is sliced
-// once.  The product kernel
-// stages 64 x 64 digit tiles of all S digits of op(A) and op(B) through two
-// LDS buffers (one barrier per 64-wide k chunk), runs the three real products
-// one after another over k, and folds each into double accumulators.
+// once.  The product kernel stages 64 x 64 digit tiles of all S digits of
+// op(A) and op(B) through LDS, runs the three real products one after another
+// over k, and folds each into double accumulators.  Two versions: one LDS
+// buffer with two workgroups per CU (the default), and two LDS buffers with
+// one workgroup per CU (selectable; identical bits).
 //
 // int8 MFMA 16x16x64 operands: lane l supplies 16 consecutive k of row l % 16
 // (A) or column l % 16 (B) starting at k = 16 (l / 16).  Any assignment of k to
@@ -283,8 +284,10 @@ struct ProdArgs {
   int Kp;
 };
 
-// product kernel: 256 threads = 4 waves (2 x 2), workgroup tile 64 x 64,
-// wave tile 32 x 32 = 2 x 2 MFMA tiles; S digit planes per operand.
+// product kernel (two LDS buffers, one workgroup per CU): 256 threads = 4
+// waves (2 x 2), workgroup tile 64 x 64, wave tile 32 x 32 = 2 x 2 MFMA tiles;
+// S digit planes per operand; the next chunk's global loads are issued before
+// the current chunk's MFMAs and stored to the other buffer after them.
 template <int S>
 __global__ __launch_bounds__(256) void gemm_i8x_kernel(GemmArgs g, ProdArgs o, int tiles_m,
                                                        int tiles_n) {
