@@ -42,7 +42,9 @@ extern "C" {
 #define SINV_PREC_MIXED 1 /* complex64 GEMMs on FP32 MFMA, FP64 inverses    */
 #define SINV_PREC_INT8X 2 /* complex128 GEMMs from 7 int8 digit planes per   */
                           /* operand on the int8 matrix cores (exact int32   */
-                          /* sums, ~2^-48 relative), FP64 inverses; RGF only */
+                          /* digit sums, error <= 2^-44 K max|row| max|col|  */
+                          /* per real product), FP64 inverses; RGF and the   */
+                          /* adjoint (other drivers: SINV_E_INVAL)           */
 
 #define SINV_OP_N 0
 #define SINV_OP_T 1
