@@ -43,8 +43,9 @@ extern "C" {
 #define SINV_PREC_INT8X 2 /* complex128 GEMMs from 7 int8 digit planes per   */
                           /* operand on the int8 matrix cores (exact int32   */
                           /* digit sums, error <= 2^-44 K max|row| max|col|  */
-                          /* per real product), FP64 inverses; RGF and the   */
-                          /* adjoint (other drivers: SINV_E_INVAL)           */
+                          /* per real product), FP64 inverses; taken by the  */
+                          /* RGF and adjoint drivers; sinv_zgemm_batched     */
+                          /* rejects it (sinv_zgemm_int8x_batched instead)   */
 
 #define SINV_OP_N 0
 #define SINV_OP_T 1
