@@ -302,7 +302,7 @@ int sinv_zgemm_batched(int M, int N, int K, int opA, int opB, sinv_c128 alpha,
 /* ------------------------------------------------------------------------
  * Complex128 GEMM from int8 digit planes (no counterpart in the reference,
  * whose `@` is FP64 BLAS; it stands in for the same products, e.g.
- * rgf.py:47,52,58,60).  Same contract as sinv_zgemm_batched.  Each row of
+ * rgf.py:48,53,60,62).  Same contract as sinv_zgemm_batched.  Each row of
  * op(A) and column of op(B) is scaled by a power of two and split into
  * `slices` (5..7) signed 7-bit digits per real part (re, im, re + im); digit
  * products are summed exactly in int32 on the int8 matrix cores and pairs
