@@ -247,3 +247,27 @@ def test_rgf_int8x_graph_replays_match_eager(built_lib):
             assert torch.equal(a, r), rep
         plan.check()
         del other
+
+
+def test_autograd_dos_gradient_int8x(built_lib):
+    """dL/dH of the DOS loss through the autograd Function with int8x GEMMs,
+    several energies near the band (eta = 1e-3): loss and gradient within the
+    1e-8 gate of the FP64 path."""
+    import torch
+    from sinv_amd import autograd as sag
+    from sinv_amd.utils import configs
+    H = configs.hermitian_H(10, 96, seed=21)
+    E = configs.energies(6, seed=22)
+    dev = torch.device("cuda")
+    out = {}
+    for prec in ("fp64", "int8x"):
+        Hd, Hu, Hl = (torch.tensor(x, device=dev, requires_grad=True) for x in H)
+        Gd, Gu, Gl = sag.greens_function(Hd, Hu, Hl, E, 1e-3, precision=prec, checkpoint_every=4)
+        L = sag.dos_loss(Gd)
+        L.backward()
+        out[prec] = (L.item(), [x.grad.detach() for x in (Hd, Hu, Hl)])
+    (l64, g64), (l8, g8) = out["fp64"], out["int8x"]
+    assert abs(l8 - l64) <= 1e-8 * abs(l64)
+    num = sum(float(((a - b).abs() ** 2).sum()) for a, b in zip(g8, g64))
+    den = sum(float((b.abs() ** 2).sum()) for b in g64)
+    assert np.sqrt(num / den) < 1e-8, np.sqrt(num / den)
