@@ -98,6 +98,8 @@ def _kind_summary(res, kind):
     if kind == "gemm_i8x":
         d["int8_tops_executed"] = round(tf * I8X_OPS_PER_FLOP, 1)
         d["int8_peak_frac"] = round(tf * I8X_OPS_PER_FLOP / PEAK_TFLOPS["gemm_i8x"], 4)
+        d["timed"] = ("each call's digit-split and product kernels together (event pair around the "
+                      "call), so the int8 rate is a lower bound for the product kernel alone")
     return d
 
 
