@@ -729,6 +729,27 @@ def bench_gemm(args):
     dev = core.device()
     gen = torch.Generator(device=dev).manual_seed(7)
     rows = []
+    if args.traffic_pass:
+        # one call per precision at b = 384 (default tile config) and a 2 GiB
+        # calibration copy, untimed, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
+        b, nb = 384, 256
+        A = torch.complex(torch.randn(nb, b, b, dtype=torch.float64, device=dev, generator=gen),
+                          torch.randn(nb, b, b, dtype=torch.float64, device=dev, generator=gen))
+        H = torch.complex(torch.randn(1, b, b, dtype=torch.float64, device=dev, generator=gen),
+                          torch.randn(1, b, b, dtype=torch.float64, device=dev, generator=gen))
+        for prec in ("fp64", "int8x"):
+            core.zgemm_batched(A, H, precision=prec)
+        torch.cuda.synchronize()
+        del A, H
+        n = 1 << 28
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(traffic_pass="zgemm_headline_shapes", b=b, products=nb,
+                              calib_kernel="neg", calib_read_bytes=8 * n,
+                              calib_write_bytes=8 * n)), flush=True)
+        return 0
     for b in (128, 256, 384):
         nb = 256
         A = torch.complex(torch.randn(nb, b, b, dtype=torch.float64, device=dev, generator=gen),
