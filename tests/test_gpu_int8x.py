@@ -271,3 +271,30 @@ def test_autograd_dos_gradient_int8x(built_lib):
     num = sum(float(((a - b).abs() ** 2).sum()) for a, b in zip(g8, g64))
     den = sum(float((b.abs() ** 2).sum()) for b in g64)
     assert np.sqrt(num / den) < 1e-8, np.sqrt(num / den)
+
+
+def test_planset_int8x_concurrent_matches_sequential(built_lib):
+    """int8x plans of different block sizes run concurrently on their own
+    streams (each with its own digit-plane workspace) give the same bits as
+    the plans run one after another."""
+    import torch
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    plans = []
+    for gi, b in enumerate((40, 64, 100)):
+        H = configs.hermitian_H(6, b, seed=60 + gi)
+        E = configs.energies(4, seed=70 + gi) + 1e-3j
+        p = core.RgfPlan(*H, shift=E, dscale=-1.0, oscale=-1.0, precision="int8x")
+        p.run()
+        plans.append(p)
+    torch.cuda.synchronize()
+    refs = [tuple(x.clone() for x in p.outputs()) for p in plans]
+    for p in plans:
+        for x in p.outputs():
+            x.zero_()
+    core.PlanSet(plans).run()
+    torch.cuda.synchronize()
+    for p, ref in zip(plans, refs):
+        for a, r in zip(p.outputs(), ref):
+            assert torch.equal(a, r)
+        p.check()
