@@ -223,7 +223,7 @@ def check_info(info_tensor, what="inverse"):
         raise err
 
 
-PROF_KINDS = ("gemm_f64", "gemm_f32", "inv_leaf", "other", "gemm_i8x")
+PROF_KINDS = ("gemm_f64", "gemm_f32", "inv_leaf", "other", "gemm_i8x", "inv_panel")
 
 # leaf inverse kernels (include/sinv.h SINV_LEAF_*)
 LEAF_VARIANTS = {"auto": 1, "lds": 0, "rows8": 2, "panel8": 3, "panel16": 4, "panel16f": 5,

@@ -9,14 +9,14 @@
 // (inv_leaf_kernel_rows, exact first maximum) and the older LDS-resident
 // kernel (inv_leaf_kernel) are kept as selectable alternatives.
 //
-// Larger blocks: 2 x 2 Schur-complement recursion, all of whose arithmetic is
-// batched GEMM on FP64 MFMA:
-//   X11 = inv(A11); T21 = A21 X11; T12 = X11 A12; S = A22 - A21 T12
-//   Y = inv(S); A12 <- -T12 Y; A21 <- -Y T21; A11 <- X11 - A12 T21
-// Pivoting is therefore local to each leaf: the recursion needs nonsingular
-// leading sub-blocks, which diagonally dominant blocks and blocks whose
-// anti-Hermitian part is definite (the Schur complements of (E + i eta) I - H,
-// eta > 0) always have.  Cost 8 n^3 real flops either way.
+// Larger blocks (64 < n <= kMaxInverseN): Gauss-Jordan elimination with
+// partial pivoting over all n rows (exact first maximum of |re| + |im| in
+// position order, as izamax), blocked by columns: tall panels of 64, 32, 16
+// or 8 columns are eliminated over all rows by a register-resident kernel
+// (an L2-resident one above 2048 rows), and each panel's elimination reaches
+// the other columns through batched GEMMs on FP64 MFMA.  Cost 8 n^3 real
+// flops either way.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -658,24 +658,648 @@ int leaf_variant() {
 }
 constexpr int kLeafPanel8MinGrid = 512;  // 2 x the 256 CUs of MI355X
 
-// Split so that the recursion's GEMM dimensions stay multiples of the 64-wide
-// GEMM tile where possible: the largest multiple of 64 not above n/2, else a
-// multiple of 16 near n/2.
-int split_point(int n) {
-  int n1 = (n / 2) / 64 * 64;
-  if (n1 == 0) n1 = ((n / 2 + 15) / 16) * 16;
-  if (n1 >= n || n1 <= 0) n1 = n / 2;
-  return n1;
+// ---------------------------------------------------------------------------
+// Blocks larger than the leaf: Gauss-Jordan elimination with partial pivoting
+// over all n rows, blocked by columns.
+//
+// Rows never move.  Row r carries pos[r], its position in the row order that a
+// swapping elimination would have.  The pivot of column k is the first
+// maximum of |re| + |im|, in position order, among the rows with pos >= k (the
+// rows not yet used as pivots): izamax's choice inside LAPACK's zgetrf, which
+// numpy.linalg.inv calls.  The pivot row takes position k and the row that
+// held position k takes the pivot's old position.
+//
+// Eliminating a set of columns J with pivot rows R = piv(J) in place (the
+// eliminated columns replaced by the inverse's columns) changes every other
+// column c as
+//     A[:, c] <- A[:, J] A[R, c] + (A[:, c] with rows R cleared)
+// with A[:, J] the eliminated panel.  A column range is therefore eliminated
+// recursively,
+//     GJ(J1);  B = A[piv(J1), J2], A[piv(J1), J2] = 0;  A[:, J2] += A[:, J1] B
+//     GJ(J2);  B = A[piv(J2), J1], A[piv(J2), J1] = 0;  A[:, J1] += A[:, J2] B
+// with the same pivots as column-by-column elimination.  The products are
+// batched GEMMs on FP64 MFMA (n x n2 x n1 each); a range of at most one panel
+// is eliminated over all n rows by a tall-panel kernel.  Panels and products
+// together are n^3 complex multiply-adds (8 n^3 flops).  The inverse is then
+// E[piv[r]][pos[s]] at (r, s), E the eliminated matrix: one permuted copy.
+
+// Tall panel in registers: 8 waves; lane l holds rows l + 64 s (s < M), wave
+// v holds the panel's columns v R .. v R + R - 1.  The wave owning a column
+// finds its pivot (per-lane maximum over its M rows, then a wave-wide maximum
+// of the modulus bits in three 32-bit DPP passes -- high word, low word, then
+// the smallest position among exact ties), forms the multipliers
+// g = a_ik / pivot (-1 / pivot on the pivot row, whose entries are replaced
+// rather than updated) and publishes them with the pivot row's index; after
+// one barrier per R steps every other wave applies the R rank-1 updates
+// a <- keep a - g P to its columns, reading the pivot row's entries P from
+// its own registers.  Every element sees the same operations in the same
+// order as in column-by-column elimination.
+constexpr int kGjWaves = 8;
+
+// Publishes the row held by (lane plane, slot) -- both wave-uniform -- in
+// this wave's R columns to the wave's LDS slot prw (read out with readlane
+// under uniform branches, so the row array stays in registers); every lane
+// then reads the entries it needs from there (a broadcast read).
+template <int M, int R>
+__device__ __forceinline__ void gj_row(const cplx (&a)[M][R], int slot, int plane, int lane,
+                                       cplx* prw) {
+#pragma unroll
+  for (int s = 0; s < M; ++s)
+    if (s == slot) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const cplx v{readlane_f64(a[s][r].re, plane), readlane_f64(a[s][r].im, plane)};
+        if (lane == 0) prw[r] = v;
+      }
+    }
+  wave_fence();
+}
+
+// a <- a - g P, or -g P on the pivot row (whose entries are replaced)
+__device__ __forceinline__ cplx gj_upd(cplx av, cplx g, cplx P, bool is_p) {
+  const double are = is_p ? 0.0 : av.re, aim = is_p ? 0.0 : av.im;
+  return cplx{are - (g.re * P.re - g.im * P.im), aim - (g.re * P.im + g.im * P.re)};
+}
+
+// Pivot of global column kg, held in this wave's column kr: the first maximum
+// of |re| + |im| in position order among the rows with pos >= kg (positions
+// in LDS).  Returns its lane and slot (row = lane + 64 slot), its position and
+// whether its modulus is zero.  NaN moduli do not take part; a column with no
+// candidate keeps the row at position kg.  All lanes of the wave must be
+// active.
+template <int M, int R>
+__device__ __forceinline__ void gj_pivot(const cplx (&a)[M][R], const int* spos, int kr, int kg,
+                                         int n, int lane, int& lp, int& sp, int& pp,
+                                         bool& zero) {
+  double mb = -1.0;
+  int pb = 0x7fffffff, sb = 0;
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
+    const int row = lane + 64 * s;
+    const int ps = row < n ? spos[row] : -1;
+    const cplx f = a[s][kr];
+    const double m = fabs(f.re) + fabs(f.im);
+    const bool better = (ps >= kg) & (m == m) & ((m > mb) | ((m == mb) & (ps < pb)));
+    mb = better ? m : mb;
+    pb = better ? ps : pb;
+    sb = better ? s : sb;
+  }
+  // a non-negative double orders like its bit pattern: compare the high
+  // words (bit 31 set marks a candidate), then the low words, then positions
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(mb);
+  const unsigned hi = (unsigned)(bits >> 32), lo = (unsigned)bits;
+  const unsigned k1 = mb >= 0.0 ? (hi | 0x80000000u) : 0u;
+  const unsigned t1 = (unsigned)__builtin_amdgcn_readlane((int)wave_umax(k1), 63);
+  zero = false;
+  if (t1 == 0u) {
+    int hs = -1;
+#pragma unroll
+    for (int s = 0; s < M; ++s)
+      hs = (lane + 64 * s < n && spos[lane + 64 * s] == kg) ? s : hs;
+    lp = (int)__builtin_ctzll(__ballot(hs >= 0));
+    sp = __builtin_amdgcn_readlane(hs, lp);
+    pp = kg;
+    return;
+  }
+  bool tie = k1 == t1;
+  unsigned long long bal = __ballot(tie);
+  if (__builtin_popcountll(bal) > 1) {
+    const unsigned t2 = (unsigned)__builtin_amdgcn_readlane((int)wave_umax(tie ? lo : 0u), 63);
+    tie = tie & (lo == t2);
+    bal = __ballot(tie);
+    if (__builtin_popcountll(bal) > 1) {
+      const unsigned k3 = tie ? ~(unsigned)pb : 0u;
+      const unsigned t3 = (unsigned)__builtin_amdgcn_readlane((int)wave_umax(k3), 63);
+      bal = __ballot(tie & (k3 == t3));
+    }
+  }
+  lp = (int)__builtin_ctzll(bal);
+  sp = __builtin_amdgcn_readlane(sb, lp);
+  pp = __builtin_amdgcn_readlane(pb, lp);
+  zero = t1 == 0x80000000u && (unsigned)__builtin_amdgcn_readlane((int)lo, lp) == 0u;
+}
+
+// The owner wave's R elimination steps of panel kb (columns c0 + kb R + kr):
+// pivot, multipliers published to gb, own columns updated, positions moved
+// (the pivot row takes position kg, the row that held kg the pivot's old
+// position).  Loops run columns-outer when a wave holds more columns than
+// rows per lane, rows-outer otherwise, to keep the fewest values live.
+template <int M, int R, bool FULL>
+__device__ __forceinline__ void gj_factor(cplx (&a)[M][R], int* spos, int* rowat, int kb, int c0,
+                                          int wc, int n, int lane, cplx* gb, int* pb, cplx* prw,
+                                          int* ipiv, bool& singular) {
+#pragma unroll
+  for (int kr = 0; kr < R; ++kr) {
+    const int k = kb * R + kr;
+    if (FULL || k < wc) {
+      const int kg = c0 + k;
+      int lp, sp, pp;
+      bool zero;
+      gj_pivot<M, R>(a, spos, kr, kg, n, lane, lp, sp, pp, zero);
+      singular |= zero;
+      const int p = lp + 64 * sp;
+      gj_row<M, R>(a, sp, lp, lane, prw);
+      const cplx d = crecip_uniform(prw[kr]);
+      const cplx one{1.0, 0.0};
+      if constexpr (R > M) {
+        cplx g[M];
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+          const cplx gf = cmul(a[s][kr], d);
+          const bool is_p = lane + 64 * s == p;
+          g[s] = cplx{is_p ? -d.re : gf.re, is_p ? -d.im : gf.im};
+          gb[(kr * M + s) * 64 + lane] = g[s];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const cplx Pr = r == kr ? one : prw[r];
+#pragma unroll
+          for (int s = 0; s < M; ++s)
+            a[s][r] = gj_upd(r == kr ? cplx{0.0, 0.0} : a[s][r], g[s], Pr, lane + 64 * s == p);
+        }
+      } else {
+        cplx P[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) P[r] = r == kr ? one : prw[r];
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+          const cplx gf = cmul(a[s][kr], d);
+          const bool is_p = lane + 64 * s == p;
+          const cplx g{is_p ? -d.re : gf.re, is_p ? -d.im : gf.im};
+          gb[(kr * M + s) * 64 + lane] = g;
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            a[s][r] = gj_upd(r == kr ? cplx{0.0, 0.0} : a[s][r], g, P[r], is_p);
+        }
+      }
+      if (lane == 0) {
+        const int q = rowat[kg];
+        spos[q] = pp;
+        rowat[pp] = q;
+        spos[p] = kg;
+        rowat[kg] = p;
+        pb[kr] = p;
+        ipiv[kg] = p;
+      }
+      wave_fence();
+    }
+  }
+}
+
+// Every other wave: the R published steps of panel kb applied to its columns.
+template <int M, int R, bool FULL>
+__device__ __forceinline__ void gj_apply(cplx (&a)[M][R], int kb, int wc, int lane,
+                                         const cplx* gb, const int* pb, cplx* prw) {
+#pragma unroll 1
+  for (int kr = 0; kr < R; ++kr) {
+    const int k = kb * R + kr;
+    if (FULL || k < wc) {
+      const int p = __builtin_amdgcn_readfirstlane(pb[kr]);
+      gj_row<M, R>(a, p >> 6, p & 63, lane, prw);
+      if constexpr (R > M) {
+        cplx g[M];
+#pragma unroll
+        for (int s = 0; s < M; ++s) g[s] = gb[(kr * M + s) * 64 + lane];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const cplx Pr = prw[r];
+#pragma unroll
+          for (int s = 0; s < M; ++s) a[s][r] = gj_upd(a[s][r], g[s], Pr, lane + 64 * s == p);
+        }
+      } else {
+        cplx P[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) P[r] = prw[r];
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+          const cplx g = gb[(kr * M + s) * 64 + lane];
+#pragma unroll
+          for (int r = 0; r < R; ++r) a[s][r] = gj_upd(a[s][r], g, P[r], lane + 64 * s == p);
+        }
+      }
+      wave_fence();  // the slot prw is rewritten by the next step
+    }
+  }
+}
+
+// Eliminates columns [c0, c0 + wc) (wc <= 8 R) of every n x n matrix over all
+// its rows, n <= 64 M.  ipos / ipiv (n ints per matrix) carry the row
+// positions between launches (initialised here when c0 == 0) and receive the
+// pivot row of each eliminated column.
+template <int M, int R>
+__global__ __launch_bounds__(64 * kGjWaves) void gj_panel_kernel(
+    cplx* A, int n, int lda, long long s_in, long long s_out, int inner, int c0, int wc,
+    int* ipos, int* ipiv, unsigned long long* info, int tag) {
+  __shared__ cplx gbuf[2][R * M * 64];  // multipliers of the panel's steps, two panels
+  __shared__ int pbuf[2][R];             // pivot row of each step
+  __shared__ int spos[64 * M];           // position of each row
+  __shared__ int rowat[64 * M];          // row at each position
+  __shared__ cplx prws[kGjWaves][R];     // pivot row entries, one slot per wave
+  const int z = blockIdx.x;
+  const int zo = z / inner, zi = z - zo * inner;
+  A += zi * s_in + zo * s_out;
+  ipos += (long long)z * n;
+  ipiv += (long long)z * n;
+  const int lane = threadIdx.x & 63;
+  const int v = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
+  cplx a[M][R];
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
+    const int row = lane + 64 * s;
+    const cplx* src = A + (long long)row * lda + c0 + v * R;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (row < n && v * R + r < wc) {
+        const double2 t = *reinterpret_cast<const double2*>(src + r);
+        a[s][r] = cplx{t.x, t.y};
+      } else {
+        a[s][r] = cplx{0.0, 0.0};
+      }
+    }
+  }
+  for (int row = threadIdx.x; row < n; row += 64 * kGjWaves) {
+    const int ps = c0 == 0 ? row : ipos[row];
+    spos[row] = ps;
+    rowat[ps] = row;
+  }
+  __syncthreads();
+
+  bool singular = false;
+  for (int kb = 0; kb < kGjWaves && kb * R < wc; ++kb) {
+    const int buf = kb & 1;
+    const bool full = kb * R + R <= wc;
+    if (v == kb) {
+      if (full)
+        gj_factor<M, R, true>(a, spos, rowat, kb, c0, wc, n, lane, gbuf[buf], pbuf[buf],
+                              prws[v], ipiv, singular);
+      else
+        gj_factor<M, R, false>(a, spos, rowat, kb, c0, wc, n, lane, gbuf[buf], pbuf[buf],
+                               prws[v], ipiv, singular);
+    }
+    __syncthreads();
+    if (v != kb) {
+      if (full)
+        gj_apply<M, R, true>(a, kb, wc, lane, gbuf[buf], pbuf[buf], prws[v]);
+      else
+        gj_apply<M, R, false>(a, kb, wc, lane, gbuf[buf], pbuf[buf], prws[v]);
+    }
+  }
+
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
+    const int row = lane + 64 * s;
+    if (row < n) {
+      cplx* dst = A + (long long)row * lda + c0 + v * R;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (v * R + r < wc) *reinterpret_cast<double2*>(dst + r) = double2{a[s][r].re, a[s][r].im};
+    }
+  }
+  __syncthreads();  // the last owner's position updates
+  for (int row = threadIdx.x; row < n; row += 64 * kGjWaves) ipos[row] = spos[row];
+  if (lane == 0 && singular && info != nullptr)
+    atomicMin(info, ((unsigned long long)(unsigned)tag << 32) | (unsigned long long)(unsigned)z);
+}
+
+// Panels of blocks above 64 M = 2048 rows: the panel (16 columns) stays in
+// global memory, where it is L2-resident, and is walked 64 rows at a time per
+// step (16 lanes per row, lane j on column j; one workgroup per matrix).  The
+// pivot search of the next column runs inside the current step's update pass
+// on the values it writes.  Positions live in LDS (pos and its inverse), so n
+// is limited by LDS to kMaxInverseN (64 KB of positions).
+constexpr int kGjGlobalW = 16;
+
+__device__ __forceinline__ double2 shfl_c(double2 v, int src) {
+  return double2{__shfl(v.x, src), __shfl(v.y, src)};
+}
+
+// Wave winner of (modulus, position, row) candidates -> slot w of the block's
+// table (bits 0 = no candidate).  All lanes active.
+__device__ __forceinline__ void gj_wave_best(double mb, int pb, int rb, int lane, int w,
+                                             unsigned long long* wbits, int* wpos, int* wrow) {
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(mb);
+  const unsigned hi = (unsigned)(bits >> 32), lo = (unsigned)bits;
+  const unsigned k1 = mb >= 0.0 ? (hi | 0x80000000u) : 0u;
+  const unsigned t1 = (unsigned)__builtin_amdgcn_readlane((int)wave_umax(k1), 63);
+  int winner = -1;
+  if (t1 != 0u) {
+    bool tie = k1 == t1;
+    unsigned long long bal = __ballot(tie);
+    if (__builtin_popcountll(bal) > 1) {
+      const unsigned t2 = (unsigned)__builtin_amdgcn_readlane((int)wave_umax(tie ? lo : 0u), 63);
+      tie = tie & (lo == t2);
+      bal = __ballot(tie);
+      if (__builtin_popcountll(bal) > 1) {
+        const unsigned k3 = tie ? ~(unsigned)pb : 0u;
+        const unsigned t3 = (unsigned)__builtin_amdgcn_readlane((int)wave_umax(k3), 63);
+        bal = __ballot(tie & (k3 == t3));
+      }
+    }
+    winner = (int)__builtin_ctzll(bal);
+  }
+  const unsigned long long wb =
+      winner < 0 ? 0ull
+                 : (1ull << 63) |
+                       ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)hi, winner)
+                        << 32) |
+                       (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)lo, winner);
+  const int wp = winner < 0 ? 0x7fffffff : __builtin_amdgcn_readlane(pb, winner);
+  const int wr = winner < 0 ? -1 : __builtin_amdgcn_readlane(rb, winner);
+  if (lane == 0) {
+    wbits[w] = wb;
+    wpos[w] = wp;
+    wrow[w] = wr;
+  }
+}
+
+// Thread 0: the block's pivot of global column kg from the wave winners (the
+// largest modulus, then the smallest position; none: the row at position kg),
+// its 1/pivot and whether the pivot is zero, into LDS.
+__device__ __forceinline__ void gj_pick(const cplx* A, int lda, int c0, int kg, int nw,
+                                        const unsigned long long* wbits, const int* wpos,
+                                        const int* wrow, const int* rowat, int* s_p, int* s_pp,
+                                        cplx* s_d, int* s_zero) {
+  unsigned long long bb = 0ull;
+  int bp = 0x7fffffff, br = -1;
+  for (int w = 0; w < nw; ++w)
+    if (wbits[w] > bb || (wbits[w] == bb && bb != 0ull && wpos[w] < bp)) {
+      bb = wbits[w];
+      bp = wpos[w];
+      br = wrow[w];
+    }
+  if (bb == 0ull) {
+    br = rowat[kg];
+    bp = kg;
+  }
+  const double2 v = *reinterpret_cast<const double2*>(A + (long long)br * lda + kg);
+  *s_p = br;
+  *s_pp = bp;
+  *s_zero = (v.x == 0.0 && v.y == 0.0) ? 1 : 0;
+  *s_d = crecip(cplx{v.x, v.y});
+  (void)c0;
+}
+
+__global__ __launch_bounds__(1024) void gj_panel_global_kernel(
+    cplx* A, int n, int lda, long long s_in, long long s_out, int inner, int c0, int wc,
+    int* ipos, int* ipiv, unsigned long long* info, int tag) {
+  extern __shared__ int gj_lds[];  // pos[n], then rowat[n] (the row at each position)
+  int* spos = gj_lds;
+  int* rowat = gj_lds + n;
+  __shared__ cplx prow[kGjGlobalW];
+  __shared__ unsigned long long wbits[16];
+  __shared__ int wpos[16], wrow[16];
+  __shared__ int s_p, s_pp, s_zero;
+  __shared__ cplx s_d;
+  const int z = blockIdx.x;
+  const int zo = z / inner, zi = z - zo * inner;
+  A += zi * s_in + zo * s_out;
+  ipos += (long long)z * n;
+  ipiv += (long long)z * n;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int j = tid & 15, grp = tid >> 4;
+  const int src0 = lane & ~15;
+
+  for (int r = tid; r < n; r += 1024) {
+    const int p = c0 == 0 ? r : ipos[r];
+    spos[r] = p;
+    rowat[p] = r;
+  }
+  __syncthreads();
+  {  // pivot of the first column
+    double mb = -1.0;
+    int pb = 0x7fffffff, rb = -1;
+    if (j == 0)
+      for (int r = grp; r < n; r += 64) {
+        const double2 f = *reinterpret_cast<const double2*>(A + (long long)r * lda + c0);
+        const double m = fabs(f.x) + fabs(f.y);
+        const int ps = spos[r];
+        const bool better = (ps >= c0) & (m == m) & ((m > mb) | ((m == mb) & (ps < pb)));
+        mb = better ? m : mb;
+        pb = better ? ps : pb;
+        rb = better ? r : rb;
+      }
+    gj_wave_best(mb, pb, rb, lane, w, wbits, wpos, wrow);
+    __syncthreads();
+    if (tid == 0) gj_pick(A, lda, c0, c0, 16, wbits, wpos, wrow, rowat, &s_p, &s_pp, &s_d, &s_zero);
+    __syncthreads();
+  }
+
+  bool singular = false;
+  for (int k = 0; k < wc; ++k) {
+    const int kg = c0 + k;
+    const int p = s_p, pp = s_pp;
+    const cplx d = s_d;
+    singular |= s_zero != 0;
+    if (tid < wc) prow[tid] = A[(long long)p * lda + c0 + tid];
+    __syncthreads();
+    const bool next = k + 1 < wc;
+    double mb = -1.0;
+    int pb = 0x7fffffff, rb = -1;
+    const cplx P = j == k ? cplx{1.0, 0.0} : (j < wc ? prow[j] : cplx{0.0, 0.0});
+    for (int r = grp; r < n; r += 64) {
+      double2* ap = reinterpret_cast<double2*>(A + (long long)r * lda + c0 + j);
+      const double2 av = j < wc ? *ap : double2{0.0, 0.0};
+      const double2 f = shfl_c(av, src0 | k);
+      const bool is_p = r == p;
+      const cplx gf = cmul(cplx{f.x, f.y}, d);
+      const cplx g{is_p ? -d.re : gf.re, is_p ? -d.im : gf.im};
+      const double keep = is_p ? 0.0 : 1.0;
+      const cplx a0 = j == k ? cplx{0.0, 0.0} : cplx{av.x, av.y};
+      const cplx nv{keep * a0.re - (g.re * P.re - g.im * P.im),
+                    keep * a0.im - (g.re * P.im + g.im * P.re)};
+      if (j < wc) *ap = double2{nv.re, nv.im};
+      if (next && j == k + 1) {  // candidate for the next column, with its new position
+        const int ps = spos[r];
+        const int np = ps == kg ? pp : (is_p ? kg : ps);
+        const double m = fabs(nv.re) + fabs(nv.im);
+        const bool better = (np > kg) & (m == m) & ((m > mb) | ((m == mb) & (np < pb)));
+        mb = better ? m : mb;
+        pb = better ? np : pb;
+        rb = better ? r : rb;
+      }
+    }
+    if (next) gj_wave_best(mb, pb, rb, lane, w, wbits, wpos, wrow);
+    __syncthreads();
+    if (tid == 0) {
+      ipiv[kg] = p;
+      const int q = rowat[kg];
+      spos[q] = pp;
+      rowat[pp] = q;
+      spos[p] = kg;
+      rowat[kg] = p;
+      if (next) gj_pick(A, lda, c0, kg + 1, 16, wbits, wpos, wrow, rowat, &s_p, &s_pp, &s_d, &s_zero);
+    }
+    __syncthreads();
+  }
+  for (int r = tid; r < n; r += 1024) ipos[r] = spos[r];
+  if (tid == 0 && singular && info != nullptr)
+    atomicMin(info, ((unsigned long long)(unsigned)tag << 32) | (unsigned long long)(unsigned)z);
+}
+
+// B[z][t][c] = A[z][piv[k0 + t]][c0 + c], and that entry of A cleared
+// (t < K, c < N; B compact K x N per matrix).
+__global__ __launch_bounds__(256) void gj_extract_kernel(cplx* A, int lda, long long s_in,
+                                                         long long s_out, int inner,
+                                                         const int* ipiv, int n, int k0, int K,
+                                                         int c0, int N, cplx* B) {
+  const int z = blockIdx.y;
+  const int zo = z / inner, zi = z - zo * inner;
+  A += zi * s_in + zo * s_out;
+  ipiv += (long long)z * n + k0;
+  const long long total = (long long)K * N;
+  double2* Bz = reinterpret_cast<double2*>(B + (long long)z * total);
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (long long)gridDim.x * 256) {
+    const int t = (int)(e / N), c = (int)(e - (long long)t * N);
+    double2* src = reinterpret_cast<double2*>(A + (long long)ipiv[t] * lda + c0 + c);
+    Bz[e] = *src;
+    *src = double2{0.0, 0.0};
+  }
+}
+
+// W[z] = A[z] (compact n x n copy)
+__global__ __launch_bounds__(256) void gj_copy_kernel(const cplx* A, int n, int lda,
+                                                      long long s_in, long long s_out, int inner,
+                                                      cplx* W) {
+  const int z = blockIdx.y;
+  const int zo = z / inner, zi = z - zo * inner;
+  A += zi * s_in + zo * s_out;
+  const long long total = (long long)n * n;
+  double2* Wz = reinterpret_cast<double2*>(W + (long long)z * total);
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (long long)gridDim.x * 256) {
+    const int r = (int)(e / n), c = (int)(e - (long long)r * n);
+    Wz[e] = *reinterpret_cast<const double2*>(A + (long long)r * lda + c);
+  }
+}
+
+// A[z][r][s] = W[z][piv[r]][pos[s]]: the inverse from the eliminated matrix
+__global__ __launch_bounds__(256) void gj_permute_kernel(cplx* A, int n, int lda, long long s_in,
+                                                         long long s_out, int inner,
+                                                         const cplx* W, const int* ipiv,
+                                                         const int* ipos) {
+  const int r = blockIdx.x, z = blockIdx.y;
+  const int zo = z / inner, zi = z - zo * inner;
+  A += zi * s_in + zo * s_out;
+  const int* pos = ipos + (long long)z * n;
+  const double2* src =
+      reinterpret_cast<const double2*>(W + (long long)z * n * n + (long long)ipiv[(long long)z * n + r] * n);
+  double2* dst = reinterpret_cast<double2*>(A + (long long)r * lda);
+  for (int c = threadIdx.x; c < n; c += 256) dst[c] = src[pos[c]];
+}
+
+// Tall-panel configuration for n rows: M rows per lane (the smallest
+// instantiated M >= n / 64) and R columns per wave, so that the panel holds
+// at most 16 complex values per lane up to 1024 rows (32 above); M = 0 selects
+// the L2-resident kernel.
+struct GjShape {
+  int M, R;
+};
+GjShape gj_shape(int n) {
+  static constexpr int kMs[] = {2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32};
+  const int m = (n + 63) / 64;
+  for (int M : kMs)
+    if (M >= m) return GjShape{M, M <= 2 ? 8 : (M <= 4 ? 4 : (M <= 8 ? 2 : 1))};
+  return GjShape{0, 0};
+}
+int gj_width(int n) {
+  const GjShape g = gj_shape(n);
+  return g.M > 0 ? kGjWaves * g.R : kGjGlobalW;
+}
+
+template <int M, int R>
+void gj_panel_mr(cplx* A, int n, int lda, long long s_in, long long s_out, int inner, unsigned nb,
+                 int c0, int wc, int* ipos, int* ipiv, unsigned long long* info, int tag,
+                 hipStream_t s) {
+  hipLaunchKernelGGL((gj_panel_kernel<M, R>), dim3(nb), dim3(64 * kGjWaves), 0, s, A, n, lda,
+                     s_in, s_out, inner, c0, wc, ipos, ipiv, info, tag);
+}
+
+hipError_t launch_gj_panel(cplx* A, int n, int lda, long long s_in, long long s_out, int inner,
+                           int outer, int c0, int wc, int* ipos, int* ipiv,
+                           unsigned long long* info, int tag, hipStream_t s) {
+  const unsigned nb = (unsigned)(inner * outer);
+  const int pid = prof_open(s);
+#define SINV_GJ_CASE(M_, R_)                                                           \
+  case M_:                                                                             \
+    gj_panel_mr<M_, R_>(A, n, lda, s_in, s_out, inner, nb, c0, wc, ipos, ipiv, info, tag, \
+                        s);                                                            \
+    break;
+  switch (gj_shape(n).M) {
+    SINV_GJ_CASE(2, 8)
+    SINV_GJ_CASE(3, 4)
+    SINV_GJ_CASE(4, 4)
+    SINV_GJ_CASE(5, 2)
+    SINV_GJ_CASE(6, 2)
+    SINV_GJ_CASE(7, 2)
+    SINV_GJ_CASE(8, 2)
+    SINV_GJ_CASE(10, 1)
+    SINV_GJ_CASE(12, 1)
+    SINV_GJ_CASE(14, 1)
+    SINV_GJ_CASE(16, 1)
+    SINV_GJ_CASE(20, 1)
+    SINV_GJ_CASE(24, 1)
+    SINV_GJ_CASE(28, 1)
+    SINV_GJ_CASE(32, 1)
+    default:
+      hipLaunchKernelGGL(gj_panel_global_kernel, dim3(nb), dim3(1024), 2 * n * sizeof(int), s, A,
+                         n, lda, s_in, s_out, inner, c0, wc, ipos, ipiv, info, tag);
+  }
+#undef SINV_GJ_CASE
+  prof_close(pid, s, PROF_INV_PANEL, 8.0 * n * (double)wc * wc * nb,
+             2.0 * sizeof(cplx) * n * (double)wc * nb);
+  return hipGetLastError();
+}
+
+// The elimination of columns [k0, k0 + K) applied to columns [cN, cN + N):
+// B = A[piv, cN..] (rows cleared), then A[:, cN..] += A[:, k0..] B.
+hipError_t gj_sweep(cplx* A, int n, int lda, long long s_in, long long s_out, int inner,
+                    int outer, int k0, int K, int cN, int N, const int* ipiv, cplx* B,
+                    hipStream_t s) {
+  const long long nb = (long long)inner * outer;
+  const long long total = (long long)K * N;
+  const unsigned gx = (unsigned)std::min<long long>((total + 255) / 256, 64);
+  hipLaunchKernelGGL(gj_extract_kernel, dim3(gx, (unsigned)nb), dim3(256), 0, s, A, lda, s_in,
+                     s_out, inner, ipiv, n, k0, K, cN, N, B);
+  SINV_HIP_RET(hipGetLastError());
+  GemmArgs g{};
+  g.M = n; g.N = N; g.K = K; g.opA = OP_N; g.opB = OP_N;
+  g.A = A + k0; g.lda = lda; g.sA_in = s_in; g.sA_out = s_out;
+  g.B = B; g.ldb = N; g.sB_in = total; g.sB_out = total * inner;
+  g.C = A + cN; g.ldc = lda; g.sC_in = s_in; g.sC_out = s_out;
+  g.C0 = A + cN; g.ldc0 = lda; g.sC0_in = s_in; g.sC0_out = s_out;
+  g.alpha = cplx{1.0, 0.0};
+  g.beta = cplx{1.0, 0.0};
+  g.shift = nullptr;
+  g.inner = inner; g.outer = outer;
+  return launch_gemm(g, Prec::F64, s);
+}
+
+hipError_t gj_range(cplx* A, int n, int lda, long long s_in, long long s_out, int inner,
+                    int outer, int c0, int c1, int wp, cplx* B, int* ipos, int* ipiv,
+                    unsigned long long* info, int tag, hipStream_t s) {
+  const int nc = c1 - c0;
+  if (nc <= wp)
+    return launch_gj_panel(A, n, lda, s_in, s_out, inner, outer, c0, nc, ipos, ipiv, info, tag,
+                           s);
+  const int n1 = std::max(wp, (nc / 2) / wp * wp);
+  const int mid = c0 + n1, n2 = nc - n1;
+  SINV_HIP_RET(gj_range(A, n, lda, s_in, s_out, inner, outer, c0, mid, wp, B, ipos, ipiv, info,
+                        tag, s));
+  SINV_HIP_RET(gj_sweep(A, n, lda, s_in, s_out, inner, outer, c0, n1, mid, n2, ipiv, B, s));
+  SINV_HIP_RET(gj_range(A, n, lda, s_in, s_out, inner, outer, mid, c1, wp, B, ipos, ipiv, info,
+                        tag, s));
+  return gj_sweep(A, n, lda, s_in, s_out, inner, outer, mid, n2, c0, n1, ipiv, B, s);
 }
 
 }  // namespace
 
 size_t inverse_workspace_elems(int n, int batch) {
   if (n <= LEAF) return 0;
-  const int n1 = split_point(n), n2 = n - n1;
-  const size_t own = 2ull * (size_t)n1 * (size_t)n2 * (size_t)batch;
-  const size_t a = inverse_workspace_elems(n1, batch), b = inverse_workspace_elems(n2, batch);
-  return own + (a > b ? a : b);
+  const size_t nb = (size_t)batch;
+  // eliminated-matrix copy (also the sweeps' B operand), then positions and pivots
+  return (size_t)n * n * nb + ((size_t)n * nb + 1) / 2 + 1;
 }
 
 hipError_t launch_inverse(cplx* A, int n, int lda, long long s_in, long long s_out, int inner,
@@ -718,56 +1342,22 @@ hipError_t launch_inverse(cplx* A, int n, int lda, long long s_in, long long s_o
                2.0 * sizeof(cplx) * n * (double)n * inner * (double)outer);
     return hipGetLastError();
   }
-  const int n1 = split_point(n), n2 = n - n1;
-  cplx* A11 = A;
-  cplx* A12 = A + n1;
-  cplx* A21 = A + (long long)n1 * lda;
-  cplx* A22 = A + (long long)n1 * lda + n1;
-  const long long w = (long long)n1 * n2;
-  const long long batch = (long long)inner * outer;
-  cplx* W21 = work;              // n2 x n1, ld n1
-  cplx* W12 = work + w * batch;  // n1 x n2, ld n2
-  cplx* sub = work + 2 * w * batch;
-
-  auto gemm = [&](int M, int N, int K, const cplx* Ap, int la, long long sa_in, long long sa_out,
-                  const cplx* Bp, int lb, long long sb_in, long long sb_out, cplx* Cp, int lc,
-                  long long sc_in, long long sc_out, cplx alpha, bool accumulate) {
-    GemmArgs g{};
-    g.M = M; g.N = N; g.K = K; g.opA = OP_N; g.opB = OP_N;
-    g.A = Ap; g.lda = la; g.sA_in = sa_in; g.sA_out = sa_out;
-    g.B = Bp; g.ldb = lb; g.sB_in = sb_in; g.sB_out = sb_out;
-    g.C = Cp; g.ldc = lc; g.sC_in = sc_in; g.sC_out = sc_out;
-    g.C0 = accumulate ? Cp : nullptr; g.ldc0 = lc; g.sC0_in = sc_in; g.sC0_out = sc_out;
-    g.alpha = alpha;
-    g.beta = accumulate ? cplx{1.0, 0.0} : cplx{0.0, 0.0};
-    g.shift = nullptr;
-    g.inner = inner; g.outer = outer;
-    return launch_gemm(g, Prec::F64, s);
-  };
-  const cplx one{1.0, 0.0}, mone{-1.0, 0.0};
-  const long long ws_in = w, ws_out = w * inner;
-
-  SINV_HIP_RET(launch_inverse(A11, n1, lda, s_in, s_out, inner, outer, sub, info, tag, s));
-  // T21 = A21 X11
-  SINV_HIP_RET(gemm(n2, n1, n1, A21, lda, s_in, s_out, A11, lda, s_in, s_out, W21, n1, ws_in,
-                    ws_out, one, false));
-  // T12 = X11 A12
-  SINV_HIP_RET(gemm(n1, n2, n1, A11, lda, s_in, s_out, A12, lda, s_in, s_out, W12, n2, ws_in,
-                    ws_out, one, false));
-  // S = A22 - A21 T12
-  SINV_HIP_RET(gemm(n2, n2, n1, A21, lda, s_in, s_out, W12, n2, ws_in, ws_out, A22, lda, s_in,
-                    s_out, mone, true));
-  SINV_HIP_RET(launch_inverse(A22, n2, lda, s_in, s_out, inner, outer, sub, info, tag, s));
-  // A12 = -T12 Y
-  SINV_HIP_RET(gemm(n1, n2, n2, W12, n2, ws_in, ws_out, A22, lda, s_in, s_out, A12, lda, s_in,
-                    s_out, mone, false));
-  // A21 = -Y T21
-  SINV_HIP_RET(gemm(n2, n1, n2, A22, lda, s_in, s_out, W21, n1, ws_in, ws_out, A21, lda, s_in,
-                    s_out, mone, false));
-  // A11 = X11 - A12 T21
-  SINV_HIP_RET(gemm(n1, n1, n2, A12, lda, s_in, s_out, W21, n1, ws_in, ws_out, A11, lda, s_in,
-                    s_out, mone, true));
-  return hipSuccess;
+  if (n > kMaxInverseN || work == nullptr) return hipErrorInvalidValue;
+  const long long nb = (long long)inner * outer;
+  cplx* W = work;
+  int* ipos = reinterpret_cast<int*>(work + (long long)n * n * nb);
+  int* ipiv = ipos + (long long)n * nb;
+  SINV_HIP_RET(gj_range(A, n, lda, s_in, s_out, inner, outer, 0, n, gj_width(n), W, ipos, ipiv,
+                        info, tag, s));
+  const long long total = (long long)n * n;
+  const unsigned gx = (unsigned)std::min<long long>((total + 255) / 256, 256);
+  const int pid = prof_open(s);
+  hipLaunchKernelGGL(gj_copy_kernel, dim3(gx, (unsigned)nb), dim3(256), 0, s, A, n, lda, s_in,
+                     s_out, inner, W);
+  hipLaunchKernelGGL(gj_permute_kernel, dim3((unsigned)n, (unsigned)nb), dim3(256), 0, s, A, n,
+                     lda, s_in, s_out, inner, W, ipiv, ipos);
+  prof_close(pid, s, PROF_OTHER, 0.0, 4.0 * sizeof(cplx) * total * nb);
+  return hipGetLastError();
 }
 
 }  // namespace sinv
@@ -782,6 +1372,11 @@ extern "C" int sinv_zinv_batched(void* A, int n, int lda, long long strideA, int
   using namespace sinv;
   if (A == nullptr || n < 0 || batch < 0 || lda < n) {
     set_error("sinv_zinv_batched: invalid argument");
+    return SINV_E_INVAL;
+  }
+  if (n > kMaxInverseN) {
+    set_error("sinv_zinv_batched: n = %d exceeds the largest supported block (%d)", n,
+              kMaxInverseN);
     return SINV_E_INVAL;
   }
   if (work_bytes < sinv_zinv_workspace_bytes(n, batch) || (n > kLeaf && work == nullptr)) {

@@ -67,10 +67,13 @@ hipError_t launch_transpose(int M, int N, int op, const cplx* S, int lds, long l
                             bool conj_shift = false);
 
 // K1: in-place batched inverse of n x n blocks at A + z_in*sA_in + z_out*sA_out.
-// Leaves (n <= kLeaf) use Gauss-Jordan with partial pivoting in LDS; larger n
-// use 2x2 block recursion (Schur complement) with leaves at the bottom.
-// A singular leaf pivot records key = (tag << 32 | z) into info (atomic min).
+// Gauss-Jordan elimination with partial pivoting over all n rows: one
+// register-resident workgroup per block for n <= kLeaf, column-blocked with
+// tall panels and GEMM sweeps for kLeaf < n <= kMaxInverseN (workspace from
+// inverse_workspace_elems).  A zero pivot records key = (tag << 32 | z) into
+// info (atomic min).
 constexpr int kLeaf = 64;
+constexpr int kMaxInverseN = 8192;
 size_t inverse_workspace_elems(int n, int batch);
 hipError_t launch_inverse(cplx* A, int n, int lda, long long sA_in, long long sA_out, int inner,
                           int outer, cplx* work, unsigned long long* info, int tag, hipStream_t s);
@@ -156,7 +159,8 @@ enum ProfKind : int {
   PROF_GEMM_F32 = 1,
   PROF_INV_LEAF = 2,
   PROF_OTHER = 3,
-  PROF_GEMM_I8X = 4
+  PROF_GEMM_I8X = 4,
+  PROF_INV_PANEL = 5
 };
 int prof_open(hipStream_t s);
 void prof_close(int id, hipStream_t s, int kind, double flops, double bytes);

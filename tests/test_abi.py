@@ -47,11 +47,13 @@ def test_workspace_query_is_host_only(built_lib):
                     g_diag=ctypes.c_void_p(16), g_upper=ctypes.c_void_p(16),
                     g_lower=ctypes.c_void_p(16), save_off_diag=1)
     ws = built_lib.sinv_rgf_workspace_bytes(ctypes.byref(d))
-    # three b^2 temporaries per instance plus the inverse recursion's scratch
-    assert ws >= 3 * 384 * 384 * 16 * 256
-    assert ws < 6 * 384 * 384 * 16 * 256
+    # three b^2 temporaries per instance plus the blocked inverse's b^2 copy
+    # and its row positions and pivots (2 b ints)
+    assert ws >= 4 * 384 * 384 * 16 * 256
+    assert ws < 5 * 384 * 384 * 16 * 256
     assert built_lib.sinv_zinv_workspace_bytes(64, 10) == 0
-    assert built_lib.sinv_zinv_workspace_bytes(128, 1) == 2 * 64 * 64 * 16
+    assert built_lib.sinv_zinv_workspace_bytes(128, 1) == (128 * 128 + 64 + 1) * 16
+    assert built_lib.sinv_zinv_workspace_bytes(128, 3) == (3 * 128 * 128 + 192 + 1) * 16
 
 
 def test_gemm_argument_validation(built_lib):

@@ -67,10 +67,11 @@ def test_zinv_dominant(built_lib, n):
     assert err < 1e-14, err
 
 
-@pytest.mark.parametrize("n", [2, 5, 16, 33, 64])
+@pytest.mark.parametrize("n", [2, 5, 16, 33, 64, 65, 96, 128, 200, 384, 416])
 def test_zinv_needs_pivoting(built_lib, n):
-    """Leaves pivot like LAPACK: a random non-dominant block and one with a
-    zero leading entry invert to numpy's result."""
+    """Every block size pivots like LAPACK, over all rows: a random
+    non-dominant block, one with a zero leading entry and a permutation invert
+    to numpy's result."""
     from sinv_amd import core
     rng = np.random.default_rng(100 + n)
     A = _rand(rng, 3, n, n)
