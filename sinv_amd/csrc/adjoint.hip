@@ -193,19 +193,28 @@ extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* wo
   // forward step i of RGF on B with tangents: (g_prev, dg_prev) -> (g_out, dg_out)
   auto fwd_step = [&](int i, cplx* gp, cplx* dgp, cplx* go, cplx* dgo) -> hipError_t {
     const int bi = bs(i);
+    // blocks above the leaf size form M in Bd and invert it into g (the
+    // elimination works in Bd); smaller ones invert in place
+    const bool oop = bi > kLeaf;
+    auto invert = [&](int tag) -> hipError_t {
+      if (oop)
+        return launch_inverse_to(Bd, bi, bi, 0, WI, 1, nb, go, bi, 0, WI, winv, d->info, tag, s);
+      return launch_inverse(go, bi, bi, 0, WI, 1, nb, winv, d->info, tag, s);
+    };
     // B_ii = conj(dscale) D_ii^H + conj(shift) I
-    cplx* bdst = i == 0 ? go : Bd;
+    cplx* bdst = i == 0 && !oop ? go : Bd;
     SINV_HIP_RET(launch_transpose(bi, bi, OP_C, D + Ly.doff[i], bi, 0, d->diag_stride, bdst, bi,
                                   0, WI, 1, nb, s, cds, (const cplx*)d->shift, true));
     if (i == 0) {
-      SINV_HIP_RET(launch_inverse(go, bi, bi, 0, WI, 1, nb, winv, d->info, 0, s));
+      SINV_HIP_RET(invert(0));
       SINV_HIP_RET(mm(bi, bi, bi, ws(go, bi), Tdb(0), Wt, WI, bi, one, false));
       return mm(bi, bi, bi, ws(Wt, bi), ws(go, bi), dgo, WI, bi, mone, false);
     }
     const int bp = bs(i - 1);
     // Y = B_{i,i-1} g_{i-1}  (b_i x b_{i-1});  M = B_ii - Y B_{i-1,i}
     SINV_HIP_RET(mm(bi, bp, bp, Ublk(i - 1), ws(gp, bp), Y, WI, bp, cos_, false));
-    SINV_HIP_RET(gemm(bi, bi, bp, ws(Y, bp), Lblk(i - 1), go, WI, bi, mcos, Bd, WI));
+    SINV_HIP_RET(
+        gemm(bi, bi, bp, ws(Y, bp), Lblk(i - 1), oop ? Bd : go, WI, bi, mcos, Bd, WI));
     // dM = T_ii - (T_{i,i-1} g_{i-1} + B_{i,i-1} dg_{i-1}) B_{i-1,i} - Y T_{i-1,i}
     SINV_HIP_RET(mm(bi, bp, bp, Tlb(i - 1), ws(gp, bp), Qf, WI, bp, one, false));
     SINV_HIP_RET(mm(bi, bp, bp, Ublk(i - 1), ws(dgp, bp), Qf, WI, bp, cos_, true));
@@ -213,7 +222,7 @@ extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* wo
                       d->gbar_diag_stride));
     SINV_HIP_RET(mm(bi, bi, bp, ws(Y, bp), Tub(i - 1), dM, WI, bi, mone, true));
     // g_i = M^-1, dg_i = -g_i dM g_i
-    SINV_HIP_RET(launch_inverse(go, bi, bi, 0, WI, 1, nb, winv, d->info, i, s));
+    SINV_HIP_RET(invert(i));
     SINV_HIP_RET(mm(bi, bi, bi, ws(go, bi), ws(dM, bi), Wt, WI, bi, one, false));
     return mm(bi, bi, bi, ws(Wt, bi), ws(go, bi), dgo, WI, bi, mone, false);
   };

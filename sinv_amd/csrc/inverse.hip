@@ -696,52 +696,45 @@ constexpr int kLeafPanel8MinGrid = 512;  // 2 x the 256 CUs of MI355X
 // order as in column-by-column elimination.
 constexpr int kGjWaves = 8;
 
-// Publishes the row held by (lane plane, slot) -- both wave-uniform -- in
-// this wave's R columns to the wave's LDS slot prw (read out with readlane
-// under uniform branches, so the row array stays in registers); every lane
-// then reads the entries it needs from there (a broadcast read).
+// Entries of the row at (lane plane, slot) in this wave's R columns, read
+// with readlane under branches on the wave-uniform slot.
 template <int M, int R>
-__device__ __forceinline__ void gj_row(const cplx (&a)[M][R], int slot, int plane, int lane,
-                                       cplx* prw) {
+__device__ __forceinline__ void gj_row(const cplx (&a)[M][R], int slot, int plane, cplx (&P)[R]) {
 #pragma unroll
   for (int s = 0; s < M; ++s)
     if (s == slot) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const cplx v{readlane_f64(a[s][r].re, plane), readlane_f64(a[s][r].im, plane)};
-        if (lane == 0) prw[r] = v;
-      }
+      for (int r = 0; r < R; ++r)
+        P[r] = cplx{readlane_f64(a[s][r].re, plane), readlane_f64(a[s][r].im, plane)};
     }
-  wave_fence();
 }
 
-// a <- a - g P, or -g P on the pivot row (whose entries are replaced)
-__device__ __forceinline__ cplx gj_upd(cplx av, cplx g, cplx P, bool is_p) {
-  const double are = is_p ? 0.0 : av.re, aim = is_p ? 0.0 : av.im;
-  return cplx{are - (g.re * P.re - g.im * P.im), aim - (g.re * P.im + g.im * P.re)};
+// a - g P in four FMAs, with a taken as zero on the pivot row (whose entries
+// are replaced: g = -1 / pivot there, so the row becomes P / pivot)
+__device__ __forceinline__ cplx gj_upd(cplx a, cplx g, cplx P, bool is_p) {
+  const double are = is_p ? 0.0 : a.re, aim = is_p ? 0.0 : a.im;
+  return cplx{fma(-g.re, P.re, fma(g.im, P.im, are)), fma(-g.re, P.im, fma(-g.im, P.re, aim))};
 }
 
 // Pivot of global column kg, held in this wave's column kr: the first maximum
-// of |re| + |im| in position order among the rows with pos >= kg (positions
-// in LDS).  Returns its lane and slot (row = lane + 64 slot), its position and
-// whether its modulus is zero.  NaN moduli do not take part; a column with no
-// candidate keeps the row at position kg.  All lanes of the wave must be
-// active.
+// of |re| + |im| in position order among the rows with pos >= kg.  Returns
+// its lane and slot (row = lane + 64 slot), its position and whether its
+// modulus is zero.  NaN moduli do not take part; a column with no candidate
+// keeps the row at position kg.  All lanes of the wave must be active.
 template <int M, int R>
-__device__ __forceinline__ void gj_pivot(const cplx (&a)[M][R], const int* spos, int kr, int kg,
-                                         int n, int lane, int& lp, int& sp, int& pp,
+__device__ __forceinline__ void gj_pivot(const cplx (&a)[M][R], const int (&pos)[M], int kr,
+                                         int kg, int n, int lane, int& lp, int& sp, int& pp,
                                          bool& zero) {
   double mb = -1.0;
   int pb = 0x7fffffff, sb = 0;
 #pragma unroll
   for (int s = 0; s < M; ++s) {
-    const int row = lane + 64 * s;
-    const int ps = row < n ? spos[row] : -1;
     const cplx f = a[s][kr];
     const double m = fabs(f.re) + fabs(f.im);
-    const bool better = (ps >= kg) & (m == m) & ((m > mb) | ((m == mb) & (ps < pb)));
+    const bool cand = (lane + 64 * s < n) & (pos[s] >= kg) & (m == m);
+    const bool better = cand & ((m > mb) | ((m == mb) & (pos[s] < pb)));
     mb = better ? m : mb;
-    pb = better ? ps : pb;
+    pb = better ? pos[s] : pb;
     sb = better ? s : sb;
   }
   // a non-negative double orders like its bit pattern: compare the high
@@ -754,8 +747,7 @@ __device__ __forceinline__ void gj_pivot(const cplx (&a)[M][R], const int* spos,
   if (t1 == 0u) {
     int hs = -1;
 #pragma unroll
-    for (int s = 0; s < M; ++s)
-      hs = (lane + 64 * s < n && spos[lane + 64 * s] == kg) ? s : hs;
+    for (int s = 0; s < M; ++s) hs = ((lane + 64 * s < n) & (pos[s] == kg)) ? s : hs;
     lp = (int)__builtin_ctzll(__ballot(hs >= 0));
     sp = __builtin_amdgcn_readlane(hs, lp);
     pp = kg;
@@ -780,14 +772,13 @@ __device__ __forceinline__ void gj_pivot(const cplx (&a)[M][R], const int* spos,
 }
 
 // The owner wave's R elimination steps of panel kb (columns c0 + kb R + kr):
-// pivot, multipliers published to gb, own columns updated, positions moved
-// (the pivot row takes position kg, the row that held kg the pivot's old
-// position).  Loops run columns-outer when a wave holds more columns than
-// rows per lane, rows-outer otherwise, to keep the fewest values live.
+// pivot, multipliers g = a_ik / pivot (-1 / pivot on the pivot row) published
+// to gb with the pivot row, own columns updated, positions moved (the pivot
+// row takes position kg, the row that held kg the pivot's old position).
 template <int M, int R, bool FULL>
-__device__ __forceinline__ void gj_factor(cplx (&a)[M][R], int* spos, int* rowat, int kb, int c0,
-                                          int wc, int n, int lane, cplx* gb, int* pb, cplx* prw,
-                                          int* ipiv, bool& singular) {
+__device__ __forceinline__ void gj_factor(cplx (&a)[M][R], int (&pos)[M], int kb, int c0, int wc,
+                                          int n, int lane, cplx* gb, int* pb, int* ppb,
+                                          int* pivl, bool& singular) {
 #pragma unroll
   for (int kr = 0; kr < R; ++kr) {
     const int k = kb * R + kr;
@@ -795,89 +786,61 @@ __device__ __forceinline__ void gj_factor(cplx (&a)[M][R], int* spos, int* rowat
       const int kg = c0 + k;
       int lp, sp, pp;
       bool zero;
-      gj_pivot<M, R>(a, spos, kr, kg, n, lane, lp, sp, pp, zero);
+      gj_pivot<M, R>(a, pos, kr, kg, n, lane, lp, sp, pp, zero);
       singular |= zero;
       const int p = lp + 64 * sp;
-      gj_row<M, R>(a, sp, lp, lane, prw);
-      const cplx d = crecip_uniform(prw[kr]);
-      const cplx one{1.0, 0.0};
-      if constexpr (R > M) {
-        cplx g[M];
+      cplx P[R];
+      gj_row<M, R>(a, sp, lp, P);
+      const cplx d = crecip_uniform(P[kr]);
+      cplx g[M];
 #pragma unroll
-        for (int s = 0; s < M; ++s) {
-          const cplx gf = cmul(a[s][kr], d);
-          const bool is_p = lane + 64 * s == p;
-          g[s] = cplx{is_p ? -d.re : gf.re, is_p ? -d.im : gf.im};
-          gb[(kr * M + s) * 64 + lane] = g[s];
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const cplx Pr = r == kr ? one : prw[r];
-#pragma unroll
-          for (int s = 0; s < M; ++s)
-            a[s][r] = gj_upd(r == kr ? cplx{0.0, 0.0} : a[s][r], g[s], Pr, lane + 64 * s == p);
-        }
-      } else {
-        cplx P[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) P[r] = r == kr ? one : prw[r];
-#pragma unroll
-        for (int s = 0; s < M; ++s) {
-          const cplx gf = cmul(a[s][kr], d);
-          const bool is_p = lane + 64 * s == p;
-          const cplx g{is_p ? -d.re : gf.re, is_p ? -d.im : gf.im};
-          gb[(kr * M + s) * 64 + lane] = g;
-#pragma unroll
-          for (int r = 0; r < R; ++r)
-            a[s][r] = gj_upd(r == kr ? cplx{0.0, 0.0} : a[s][r], g, P[r], is_p);
-        }
+      for (int s = 0; s < M; ++s) {
+        const cplx gf = cmul(a[s][kr], d);
+        const bool is_p = lane + 64 * s == p;
+        g[s] = cplx{is_p ? -d.re : gf.re, is_p ? -d.im : gf.im};
+        gb[(kr * M + s) * 64 + lane] = g[s];
+        a[s][kr] = cplx{-g[s].re, -g[s].im};  // the inverse's column: -g, d on the pivot row
       }
+#pragma unroll
+      for (int j = 1; j < R; ++j) {
+        const int r = (kr + j) % R;  // the next pivot column first
+#pragma unroll
+        for (int s = 0; s < M; ++s) a[s][r] = gj_upd(a[s][r], g[s], P[r], lane + 64 * s == p);
+      }
+#pragma unroll
+      for (int s = 0; s < M; ++s) pos[s] = pos[s] == kg ? pp : (lane + 64 * s == p ? kg : pos[s]);
       if (lane == 0) {
-        const int q = rowat[kg];
-        spos[q] = pp;
-        rowat[pp] = q;
-        spos[p] = kg;
-        rowat[kg] = p;
         pb[kr] = p;
-        ipiv[kg] = p;
+        ppb[kr] = pp;
+        pivl[k] = p;
       }
-      wave_fence();
     }
   }
 }
 
 // Every other wave: the R published steps of panel kb applied to its columns.
 template <int M, int R, bool FULL>
-__device__ __forceinline__ void gj_apply(cplx (&a)[M][R], int kb, int wc, int lane,
-                                         const cplx* gb, const int* pb, cplx* prw) {
-#pragma unroll 1
+__device__ __forceinline__ void gj_apply(cplx (&a)[M][R], int (&pos)[M], int kb, int c0, int wc,
+                                         int lane, const cplx* gb, const int* pb,
+                                         const int* ppb) {
+#pragma unroll
   for (int kr = 0; kr < R; ++kr) {
     const int k = kb * R + kr;
     if (FULL || k < wc) {
+      const int kg = c0 + k;
       const int p = __builtin_amdgcn_readfirstlane(pb[kr]);
-      gj_row<M, R>(a, p >> 6, p & 63, lane, prw);
-      if constexpr (R > M) {
-        cplx g[M];
+      const int pp = __builtin_amdgcn_readfirstlane(ppb[kr]);
+      cplx P[R];
+      gj_row<M, R>(a, p >> 6, p & 63, P);
 #pragma unroll
-        for (int s = 0; s < M; ++s) g[s] = gb[(kr * M + s) * 64 + lane];
+      for (int s = 0; s < M; ++s) {
+        const cplx g = gb[(kr * M + s) * 64 + lane];
+        const bool is_p = lane + 64 * s == p;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const cplx Pr = prw[r];
-#pragma unroll
-          for (int s = 0; s < M; ++s) a[s][r] = gj_upd(a[s][r], g[s], Pr, lane + 64 * s == p);
-        }
-      } else {
-        cplx P[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) P[r] = prw[r];
-#pragma unroll
-        for (int s = 0; s < M; ++s) {
-          const cplx g = gb[(kr * M + s) * 64 + lane];
-#pragma unroll
-          for (int r = 0; r < R; ++r) a[s][r] = gj_upd(a[s][r], g, P[r], lane + 64 * s == p);
-        }
+        for (int r = 0; r < R; ++r) a[s][r] = gj_upd(a[s][r], g, P[r], is_p);
       }
-      wave_fence();  // the slot prw is rewritten by the next step
+#pragma unroll
+      for (int s = 0; s < M; ++s) pos[s] = pos[s] == kg ? pp : (lane + 64 * s == p ? kg : pos[s]);
     }
   }
 }
@@ -892,9 +855,8 @@ __global__ __launch_bounds__(64 * kGjWaves) void gj_panel_kernel(
     int* ipos, int* ipiv, unsigned long long* info, int tag) {
   __shared__ cplx gbuf[2][R * M * 64];  // multipliers of the panel's steps, two panels
   __shared__ int pbuf[2][R];             // pivot row of each step
-  __shared__ int spos[64 * M];           // position of each row
-  __shared__ int rowat[64 * M];          // row at each position
-  __shared__ cplx prws[kGjWaves][R];     // pivot row entries, one slot per wave
+  __shared__ int ppbuf[2][R];            // its position before the step
+  __shared__ int pivl[kGjWaves * R];     // pivot rows of the launch's columns
   const int z = blockIdx.x;
   const int zo = z / inner, zi = z - zo * inner;
   A += zi * s_in + zo * s_out;
@@ -904,6 +866,7 @@ __global__ __launch_bounds__(64 * kGjWaves) void gj_panel_kernel(
   const int v = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
   cplx a[M][R];
+  int pos[M];
 #pragma unroll
   for (int s = 0; s < M; ++s) {
     const int row = lane + 64 * s;
@@ -917,32 +880,34 @@ __global__ __launch_bounds__(64 * kGjWaves) void gj_panel_kernel(
         a[s][r] = cplx{0.0, 0.0};
       }
     }
+    pos[s] = row < n ? (c0 == 0 ? row : ipos[row]) : -1;
   }
-  for (int row = threadIdx.x; row < n; row += 64 * kGjWaves) {
-    const int ps = c0 == 0 ? row : ipos[row];
-    spos[row] = ps;
-    rowat[ps] = row;
-  }
-  __syncthreads();
 
   bool singular = false;
   for (int kb = 0; kb < kGjWaves && kb * R < wc; ++kb) {
     const int buf = kb & 1;
     const bool full = kb * R + R <= wc;
     if (v == kb) {
+      // wave priorities follow the critical path: the factoring wave first,
+      // then the next panel's owner while it applies this panel
+      __builtin_amdgcn_s_setprio(3);
       if (full)
-        gj_factor<M, R, true>(a, spos, rowat, kb, c0, wc, n, lane, gbuf[buf], pbuf[buf],
-                              prws[v], ipiv, singular);
+        gj_factor<M, R, true>(a, pos, kb, c0, wc, n, lane, gbuf[buf], pbuf[buf], ppbuf[buf], pivl,
+                              singular);
       else
-        gj_factor<M, R, false>(a, spos, rowat, kb, c0, wc, n, lane, gbuf[buf], pbuf[buf],
-                               prws[v], ipiv, singular);
+        gj_factor<M, R, false>(a, pos, kb, c0, wc, n, lane, gbuf[buf], pbuf[buf], ppbuf[buf],
+                               pivl, singular);
     }
     __syncthreads();
     if (v != kb) {
-      if (full)
-        gj_apply<M, R, true>(a, kb, wc, lane, gbuf[buf], pbuf[buf], prws[v]);
+      if (v == kb + 1)
+        __builtin_amdgcn_s_setprio(2);
       else
-        gj_apply<M, R, false>(a, kb, wc, lane, gbuf[buf], pbuf[buf], prws[v]);
+        __builtin_amdgcn_s_setprio(0);
+      if (full)
+        gj_apply<M, R, true>(a, pos, kb, c0, wc, lane, gbuf[buf], pbuf[buf], ppbuf[buf]);
+      else
+        gj_apply<M, R, false>(a, pos, kb, c0, wc, lane, gbuf[buf], pbuf[buf], ppbuf[buf]);
     }
   }
 
@@ -954,10 +919,10 @@ __global__ __launch_bounds__(64 * kGjWaves) void gj_panel_kernel(
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (v * R + r < wc) *reinterpret_cast<double2*>(dst + r) = double2{a[s][r].re, a[s][r].im};
+      if (v == 0) ipos[row] = pos[s];
     }
   }
-  __syncthreads();  // the last owner's position updates
-  for (int row = threadIdx.x; row < n; row += 64 * kGjWaves) ipos[row] = spos[row];
+  if ((int)threadIdx.x < wc) ipiv[c0 + threadIdx.x] = pivl[threadIdx.x];
   if (lane == 0 && singular && info != nullptr)
     atomicMin(info, ((unsigned long long)(unsigned)tag << 32) | (unsigned long long)(unsigned)z);
 }
@@ -1174,18 +1139,18 @@ __global__ __launch_bounds__(256) void gj_copy_kernel(const cplx* A, int n, int 
   }
 }
 
-// A[z][r][s] = W[z][piv[r]][pos[s]]: the inverse from the eliminated matrix
-__global__ __launch_bounds__(256) void gj_permute_kernel(cplx* A, int n, int lda, long long s_in,
-                                                         long long s_out, int inner,
-                                                         const cplx* W, const int* ipiv,
+// D[z][r][s] = S[z][piv[r]][pos[s]]: the inverse from the eliminated matrix
+__global__ __launch_bounds__(256) void gj_permute_kernel(const cplx* S, int lds, long long ss_in,
+                                                         long long ss_out, cplx* D, int ldd,
+                                                         long long sd_in, long long sd_out, int n,
+                                                         int inner, const int* ipiv,
                                                          const int* ipos) {
   const int r = blockIdx.x, z = blockIdx.y;
   const int zo = z / inner, zi = z - zo * inner;
-  A += zi * s_in + zo * s_out;
   const int* pos = ipos + (long long)z * n;
-  const double2* src =
-      reinterpret_cast<const double2*>(W + (long long)z * n * n + (long long)ipiv[(long long)z * n + r] * n);
-  double2* dst = reinterpret_cast<double2*>(A + (long long)r * lda);
+  const double2* src = reinterpret_cast<const double2*>(
+      S + zi * ss_in + zo * ss_out + (long long)ipiv[(long long)z * n + r] * lds);
+  double2* dst = reinterpret_cast<double2*>(D + zi * sd_in + zo * sd_out + (long long)r * ldd);
   for (int c = threadIdx.x; c < n; c += 256) dst[c] = src[pos[c]];
 }
 
@@ -1302,61 +1267,99 @@ size_t inverse_workspace_elems(int n, int batch) {
   return (size_t)n * n * nb + ((size_t)n * nb + 1) / 2 + 1;
 }
 
+namespace {
+
+hipError_t launch_leaf(cplx* A, int n, int lda, long long s_in, long long s_out, int inner,
+                       int outer, unsigned long long* info, int tag, hipStream_t s) {
+  const int pid = prof_open(s);
+  const dim3 grid((unsigned)(inner * outer));
+  int v = leaf_variant();
+  if (v == SINV_LEAF_AUTO)
+    v = grid.x >= (unsigned)kLeafPanel8MinGrid ? SINV_LEAF_PANEL8_FLAGS : SINV_LEAF_PANEL16_FLAGS;
+  switch (v) {
+    case SINV_LEAF_LDS:
+      hipLaunchKernelGGL(inv_leaf_kernel, grid, dim3(256), 0, s, A, n, lda, s_in, s_out, inner,
+                         info, tag);
+      break;
+    case SINV_LEAF_ROWS8:
+      hipLaunchKernelGGL(inv_leaf_kernel_rows<8>, grid, dim3(512), 0, s, A, n, lda, s_in, s_out,
+                         inner, info, tag);
+      break;
+    case SINV_LEAF_PANEL8:
+      hipLaunchKernelGGL(inv_leaf_kernel_panel<8>, grid, dim3(512), 0, s, A, n, lda, s_in, s_out,
+                         inner, info, tag, g_leaf_spin_limit);
+      break;
+    case SINV_LEAF_PANEL16_FLAGS:
+      hipLaunchKernelGGL((inv_leaf_kernel_panel<16, true>), grid, dim3(1024), 0, s, A, n, lda,
+                         s_in, s_out, inner, info, tag, g_leaf_spin_limit);
+      break;
+    case SINV_LEAF_PANEL8_FLAGS:
+      hipLaunchKernelGGL((inv_leaf_kernel_panel<8, true>), grid, dim3(512), 0, s, A, n, lda, s_in,
+                         s_out, inner, info, tag, g_leaf_spin_limit);
+      break;
+    default:
+      hipLaunchKernelGGL(inv_leaf_kernel_panel<16>, grid, dim3(1024), 0, s, A, n, lda, s_in,
+                         s_out, inner, info, tag, g_leaf_spin_limit);
+  }
+  prof_close(pid, s, PROF_INV_LEAF, 8.0 * n * (double)n * n * inner * (double)outer,
+             2.0 * sizeof(cplx) * n * (double)n * inner * (double)outer);
+  return hipGetLastError();
+}
+
+// Blocked elimination of A in place (n > kLeaf); leaves positions and pivots
+// in the workspace after the n^2-per-matrix copy region W.
+hipError_t gj_eliminate(cplx* A, int n, int lda, long long s_in, long long s_out, int inner,
+                        int outer, cplx* work, int** ipos, int** ipiv, unsigned long long* info,
+                        int tag, hipStream_t s) {
+  const long long nb = (long long)inner * outer;
+  *ipos = reinterpret_cast<int*>(work + (long long)n * n * nb);
+  *ipiv = *ipos + (long long)n * nb;
+  return gj_range(A, n, lda, s_in, s_out, inner, outer, 0, n, gj_width(n), work, *ipos, *ipiv,
+                  info, tag, s);
+}
+
+}  // namespace
+
 hipError_t launch_inverse(cplx* A, int n, int lda, long long s_in, long long s_out, int inner,
                           int outer, cplx* work, unsigned long long* info, int tag,
                           hipStream_t s) {
   if (n <= 0 || inner <= 0 || outer <= 0) return hipSuccess;
-  if (n <= LEAF) {
-    const int pid = prof_open(s);
-    const dim3 grid((unsigned)(inner * outer));
-    int v = leaf_variant();
-    if (v == SINV_LEAF_AUTO)
-      v = grid.x >= (unsigned)kLeafPanel8MinGrid ? SINV_LEAF_PANEL8_FLAGS
-                                                 : SINV_LEAF_PANEL16_FLAGS;
-    switch (v) {
-      case SINV_LEAF_LDS:
-        hipLaunchKernelGGL(inv_leaf_kernel, grid, dim3(256), 0, s, A, n, lda, s_in, s_out, inner,
-                           info, tag);
-        break;
-      case SINV_LEAF_ROWS8:
-        hipLaunchKernelGGL(inv_leaf_kernel_rows<8>, grid, dim3(512), 0, s, A, n, lda, s_in,
-                           s_out, inner, info, tag);
-        break;
-      case SINV_LEAF_PANEL8:
-        hipLaunchKernelGGL(inv_leaf_kernel_panel<8>, grid, dim3(512), 0, s, A, n, lda, s_in,
-                           s_out, inner, info, tag, g_leaf_spin_limit);
-        break;
-      case SINV_LEAF_PANEL16_FLAGS:
-        hipLaunchKernelGGL((inv_leaf_kernel_panel<16, true>), grid, dim3(1024), 0, s, A, n, lda,
-                           s_in, s_out, inner, info, tag, g_leaf_spin_limit);
-        break;
-      case SINV_LEAF_PANEL8_FLAGS:
-        hipLaunchKernelGGL((inv_leaf_kernel_panel<8, true>), grid, dim3(512), 0, s, A, n, lda,
-                           s_in, s_out, inner, info, tag, g_leaf_spin_limit);
-        break;
-      default:
-        hipLaunchKernelGGL(inv_leaf_kernel_panel<16>, grid, dim3(1024), 0, s, A, n, lda, s_in,
-                           s_out, inner, info, tag, g_leaf_spin_limit);
-    }
-    prof_close(pid, s, PROF_INV_LEAF, 8.0 * n * (double)n * n * inner * (double)outer,
-               2.0 * sizeof(cplx) * n * (double)n * inner * (double)outer);
-    return hipGetLastError();
-  }
+  if (n <= LEAF) return launch_leaf(A, n, lda, s_in, s_out, inner, outer, info, tag, s);
   if (n > kMaxInverseN || work == nullptr) return hipErrorInvalidValue;
   const long long nb = (long long)inner * outer;
-  cplx* W = work;
-  int* ipos = reinterpret_cast<int*>(work + (long long)n * n * nb);
-  int* ipiv = ipos + (long long)n * nb;
-  SINV_HIP_RET(gj_range(A, n, lda, s_in, s_out, inner, outer, 0, n, gj_width(n), W, ipos, ipiv,
-                        info, tag, s));
+  int *ipos, *ipiv;
+  SINV_HIP_RET(gj_eliminate(A, n, lda, s_in, s_out, inner, outer, work, &ipos, &ipiv, info, tag,
+                            s));
+  // the eliminated matrix is copied aside, then permuted back into A
   const long long total = (long long)n * n;
   const unsigned gx = (unsigned)std::min<long long>((total + 255) / 256, 256);
   const int pid = prof_open(s);
   hipLaunchKernelGGL(gj_copy_kernel, dim3(gx, (unsigned)nb), dim3(256), 0, s, A, n, lda, s_in,
-                     s_out, inner, W);
-  hipLaunchKernelGGL(gj_permute_kernel, dim3((unsigned)n, (unsigned)nb), dim3(256), 0, s, A, n,
-                     lda, s_in, s_out, inner, W, ipiv, ipos);
+                     s_out, inner, work);
+  hipLaunchKernelGGL(gj_permute_kernel, dim3((unsigned)n, (unsigned)nb), dim3(256), 0, s, work, n,
+                     total, total * inner, A, lda, s_in, s_out, n, inner, ipiv, ipos);
   prof_close(pid, s, PROF_OTHER, 0.0, 4.0 * sizeof(cplx) * total * nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_inverse_to(cplx* A, int n, int lda, long long s_in, long long s_out, int inner,
+                             int outer, cplx* X, int ldx, long long sx_in, long long sx_out,
+                             cplx* work, unsigned long long* info, int tag, hipStream_t s) {
+  if (n <= 0 || inner <= 0 || outer <= 0) return hipSuccess;
+  if (n <= LEAF) {
+    SINV_HIP_RET(launch_axpby_shift(n, n, cplx{1.0, 0.0}, A, lda, s_in, s_out, X, ldx, sx_in,
+                                    sx_out, nullptr, inner, outer, s));
+    return launch_leaf(X, n, ldx, sx_in, sx_out, inner, outer, info, tag, s);
+  }
+  if (n > kMaxInverseN || work == nullptr) return hipErrorInvalidValue;
+  const long long nb = (long long)inner * outer;
+  int *ipos, *ipiv;
+  SINV_HIP_RET(gj_eliminate(A, n, lda, s_in, s_out, inner, outer, work, &ipos, &ipiv, info, tag,
+                            s));
+  const int pid = prof_open(s);
+  hipLaunchKernelGGL(gj_permute_kernel, dim3((unsigned)n, (unsigned)nb), dim3(256), 0, s, A, lda,
+                     s_in, s_out, X, ldx, sx_in, sx_out, n, inner, ipiv, ipos);
+  prof_close(pid, s, PROF_OTHER, 0.0, 2.0 * sizeof(cplx) * (double)n * n * nb);
   return hipGetLastError();
 }
 

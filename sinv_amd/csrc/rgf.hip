@@ -163,12 +163,29 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
 
   if (d->info != nullptr) SINV_HIP_TRY(launch_fill_u64(d->info, 1, SINV_INFO_OK, s));
 
-  // ---- forward sweep (rgf.py:44-48)
+  // ---- forward sweep (rgf.py:44-48).  Blocks above the leaf size form M_i in
+  // the temporary Q and invert it into the g_i slot (the elimination works in
+  // Q, so no copy of M_i is needed); smaller ones invert in place.
+  auto m_dst = [&](int i, cplx*& dst, long long& sdst) {
+    const bool oop = L.b[i] > kLeaf;
+    dst = oop ? Q : Gd + L.doff[i];
+    sdst = oop ? b2 : L.dsize;
+  };
+  auto invert_m = [&](int i) -> hipError_t {
+    const int bi = L.b[i];
+    if (bi > kLeaf)
+      return launch_inverse_to(Q, bi, bi, 0, b2, 1, nb, Gd + L.doff[i], bi, 0, L.dsize, winv,
+                               d->info, i, s);
+    return launch_inverse(Gd + L.doff[i], bi, bi, 0, L.dsize, 1, nb, winv, d->info, i, s);
+  };
   {
     const int b0 = L.b[0];
-    SINV_HIP_TRY(launch_axpby_shift(b0, b0, dscale, D + L.doff[0], b0, 0, d->diag_stride,
-                                    Gd + L.doff[0], b0, 0, L.dsize, shift, 1, nb, s));
-    SINV_HIP_TRY(launch_inverse(Gd + L.doff[0], b0, b0, 0, L.dsize, 1, nb, winv, d->info, 0, s));
+    cplx* dst;
+    long long sdst;
+    m_dst(0, dst, sdst);
+    SINV_HIP_TRY(launch_axpby_shift(b0, b0, dscale, D + L.doff[0], b0, 0, d->diag_stride, dst, b0,
+                                    0, sdst, shift, 1, nb, s));
+    SINV_HIP_TRY(invert_m(0));
   }
   for (int i = 1; i < N; ++i) {
     const int bi = L.b[i], bp = L.b[i - 1];
@@ -176,10 +193,12 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
     SINV_HIP_TRY(gemm(bi, bp, bp, Lo + L.ooff[i - 1], bp, d->lower_stride, Gd + L.doff[i - 1], bp,
                       L.dsize, P, bp, b2, one, nullptr, 0, 0, zero, nullptr, prec));
     // M = dscale D_ii + shift I - oscale^2 T upper[i-1]
-    SINV_HIP_TRY(gemm(bi, bi, bp, P, bp, b2, U + L.ooff[i - 1], bi, d->upper_stride,
-                      Gd + L.doff[i], bi, L.dsize, mosq, D + L.doff[i], bi, d->diag_stride,
-                      dscale, shift, prec));
-    SINV_HIP_TRY(launch_inverse(Gd + L.doff[i], bi, bi, 0, L.dsize, 1, nb, winv, d->info, i, s));
+    cplx* dst;
+    long long sdst;
+    m_dst(i, dst, sdst);
+    SINV_HIP_TRY(gemm(bi, bi, bp, P, bp, b2, U + L.ooff[i - 1], bi, d->upper_stride, dst, bi, sdst,
+                      mosq, D + L.doff[i], bi, d->diag_stride, dscale, shift, prec));
+    SINV_HIP_TRY(invert_m(i));
   }
 
   // ---- backward sweep (rgf.py:51-62)

@@ -77,6 +77,11 @@ constexpr int kMaxInverseN = 8192;
 size_t inverse_workspace_elems(int n, int batch);
 hipError_t launch_inverse(cplx* A, int n, int lda, long long sA_in, long long sA_out, int inner,
                           int outer, cplx* work, unsigned long long* info, int tag, hipStream_t s);
+// Out of place: A is destroyed (it is the elimination's working copy) and
+// its inverse is written to X.  Same workspace as launch_inverse.
+hipError_t launch_inverse_to(cplx* A, int n, int lda, long long sA_in, long long sA_out, int inner,
+                             int outer, cplx* X, int ldx, long long sX_in, long long sX_out,
+                             cplx* work, unsigned long long* info, int tag, hipStream_t s);
 
 // Fill n elements with a value (used for NaN off-diagonals and info reset).
 hipError_t launch_fill(cplx* p, long long n, cplx v, hipStream_t s);
