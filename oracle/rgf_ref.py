@@ -25,26 +25,34 @@ import numpy as np
 inv = np.linalg.inv
 
 
-def rgf_blocks(diag, upper, lower, sym_mat=False, save_off_diag=True):
+def rgf_blocks(diag, upper, lower, sym_mat=False, save_off_diag=True, reassociate=False):
+    """rgf.py:40-62.  ``reassociate=True`` evaluates every triple product as
+    X @ (Y @ Z) instead of the reference's (X @ Y) @ Z: an equally valid FP64
+    evaluation order, whose distance from the reference order measures the
+    instance's rounding floor (what two correct FP64 implementations can
+    differ by); it is not the reference's arithmetic."""
     N = diag.shape[0]
     Gd = np.full_like(diag, np.nan)
     Gu = np.full_like(upper, np.nan)
     Gl = np.full_like(lower, np.nan)
 
+    def mm3(X, Y, Z):
+        return X @ (Y @ Z) if reassociate else X @ Y @ Z
+
     Gd[0] = inv(diag[0])
     for i in range(1, N):
-        Gd[i] = inv(diag[i] - lower[i - 1] @ Gd[i - 1] @ upper[i - 1])
+        Gd[i] = inv(diag[i] - mm3(lower[i - 1], Gd[i - 1], upper[i - 1]))
 
     for i in range(N - 2, -1, -1):
         g_ii = Gd[i].copy()
-        X = Gd[i + 1] @ lower[i] @ g_ii
+        X = mm3(Gd[i + 1], lower[i], g_ii)
         if save_off_diag:
             Gl[i] = -X
             if sym_mat:
                 Gu[i] = Gl[i].T
             else:
-                Gu[i] = -g_ii @ upper[i] @ Gd[i + 1]
-        Gd[i] = g_ii + g_ii @ upper[i] @ X
+                Gu[i] = mm3(-g_ii, upper[i], Gd[i + 1])
+        Gd[i] = g_ii + mm3(g_ii, upper[i], X)
     return Gd, Gu, Gl
 
 

@@ -63,6 +63,40 @@ def _batched(x, nd):
     raise ValueError(f"expected a {nd}-d block stack or a batch of them, got shape {tuple(x.shape)}")
 
 
+def _shape(x):
+    """Shape of an array-like input, read without moving it to the device."""
+    return tuple(x.shape) if hasattr(x, "shape") else np.shape(x)
+
+
+def _check_stack(name, x, nblocks, rows, cols):
+    """x must be a stack of nblocks rows x cols blocks, optionally batched."""
+    s = _shape(x)
+    if len(s) not in (3, 4) or s[-3:] != (nblocks, rows, cols):
+        raise ValueError(f"{name} must hold {nblocks} blocks of {rows} x {cols} "
+                         f"(optionally with a batch dimension), got shape {s}")
+
+
+def _check_tridiag(diag, upper, lower, gbar=None):
+    """Validate a block-tridiagonal stack (and its gradient blocks) before any
+    launch: diag (N, b, b), upper / lower (N-1, b, b), each optionally batched.
+    Returns (N, b)."""
+    s = _shape(diag)
+    if len(s) not in (3, 4) or s[-1] != s[-2]:
+        raise ValueError(f"diag must be (N, b, b) or (B, N, b, b) with square blocks, got {s}")
+    N, b = s[-3], s[-1]
+    if N == 0 or b == 0:
+        raise ValueError("need at least one block of size at least 1")
+    if N > 1:
+        _check_stack("upper", upper, N - 1, b, b)
+        _check_stack("lower", lower, N - 1, b, b)
+    if gbar is not None:
+        _check_stack("gbar_diag", gbar[0], N, b, b)
+        if N > 1:
+            _check_stack("gbar_upper", gbar[1], N - 1, b, b)
+            _check_stack("gbar_lower", gbar[2], N - 1, b, b)
+    return N, b
+
+
 def _resolve_batch(*cands):
     bs = {c for c in cands if c is not None}
     if len(bs) > 1:
@@ -82,12 +116,9 @@ def rgf_batched(diag, upper, lower, *, sym_mat=False, save_off_diag=True, precis
     """
     if precision not in nat.PREC:
         raise ValueError(f"precision must be one of {sorted(nat.PREC)}")
+    _check_tridiag(diag, upper, lower)
     d, Bd, sd = _batched(to_dev(diag), 3)
     N, b = d.shape[-3], d.shape[-1]
-    if d.shape[-2] != b:
-        raise ValueError("diagonal blocks must be square")
-    if N == 0:
-        raise ValueError("need at least one block")
     if N > 1:
         u, Bu, su = _batched(to_dev(upper), 3)
         l, Bl, sl = _batched(to_dev(lower), 3)
@@ -131,6 +162,7 @@ class RgfPlan:
                  precision="fp64", shift=None, dscale=1.0, oscale=1.0, out=None):
         if precision not in nat.PREC:
             raise ValueError(f"precision must be one of {sorted(nat.PREC)}")
+        _check_tridiag(diag, upper, lower)
         self.d, Bd, sd = _batched(to_dev(diag), 3)
         N, b = self.d.shape[-3], self.d.shape[-1]
         if N > 1:
@@ -236,13 +268,12 @@ def bcr_batched(diag, upper, lower, *, shift=None, dscale=1.0, oscale=1.0, retur
     (G_diag, G_upper, G_lower) and, with ``return_fill``, the produced blocks
     of levels >= 1 as (B, F, b, b) in the order documented in include/sinv.h.
     """
+    _check_tridiag(diag, upper, lower)
     d, Bd, sd = _batched(to_dev(diag), 3)
     N, b = d.shape[-3], d.shape[-1]
     if N > 1:
         u, Bu, su = _batched(to_dev(upper), 3)
         l, Bl, sl = _batched(to_dev(lower), 3)
-        if u.shape[-3:] != (N - 1, b, b) or l.shape[-3:] != (N - 1, b, b):
-            raise ValueError("upper/lower must hold N-1 blocks of the diagonal block size")
     else:
         u = l = None
         Bu = Bl = None
@@ -287,6 +318,7 @@ def rgf_adjoint_batched(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *
     """
     if precision not in nat.PREC:
         raise ValueError(f"precision must be one of {sorted(nat.PREC)}")
+    _check_tridiag(diag, upper, lower, gbar=(gbar_diag, gbar_upper, gbar_lower))
     d, Bd, sd = _batched(to_dev(diag), 3)
     N, b = d.shape[-3], d.shape[-1]
     gd, Bgd, sgd = _batched(to_dev(gbar_diag), 3)
@@ -420,7 +452,8 @@ def rgf_adjoint_ragged(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *,
     if precision not in nat.PREC:
         raise ValueError(f"precision must be one of {sorted(nat.PREC)}")
     bs = _ragged_sizes(diag, upper, lower)
-    _ragged_sizes(gbar_diag, gbar_upper, gbar_lower)
+    if _ragged_sizes(gbar_diag, gbar_upper, gbar_lower) != bs:
+        raise ValueError("gbar blocks must have the sizes of the matrix blocks")
     N = len(bs)
     d, Bd, sd = _pack(diag)
     u, Bu, su = _pack(upper)
@@ -469,14 +502,14 @@ def psr_batched(diag, upper, lower, npart, *, shift=None, dscale=1.0, oscale=1.0
     partitions, the 2(npart-1)-block reduced system solved by RGF.  Needs
     nblocks >= 3 * npart (psr_utils.check_input).  Same instance convention and
     outputs as ``rgf_batched``."""
-    d, Bd, sd = _batched(to_dev(diag), 3)
-    N, b = d.shape[-3], d.shape[-1]
+    N, b = _check_tridiag(diag, upper, lower)
     if first_rows:
         if npart != 2 or not 1 <= first_rows < N:
             raise ValueError("an explicit first partition needs npart == 2 and 1 <= first_rows < N")
     elif N < 3 * npart or npart < 2:
         raise ValueError("The number of blocks is to low. There should be at least 3 blockrows "
                          "per partition (and at least 2 partitions)")
+    d, Bd, sd = _batched(to_dev(diag), 3)
     u, Bu, su = _batched(to_dev(upper), 3)
     l, Bl, sl = _batched(to_dev(lower), 3)
     sh = to_dev(shift).reshape(-1) if shift is not None else None
@@ -507,12 +540,12 @@ def pdiv_batched(diag, upper, lower, npart, *, check=True):
     power of 2 and at most nblocks (pdiv_utils.check_multiprocessing /
     check_input).  Same instance convention and outputs as ``rgf_batched``."""
     npart = int(npart)
-    d, Bd, sd = _batched(to_dev(diag), 3)
-    N, b = d.shape[-3], d.shape[-1]
+    N, b = _check_tridiag(diag, upper, lower)
     if npart < 1 or npart & (npart - 1):
         raise ValueError("The number of processes must be a power of 2.")
     if N < npart:
         raise ValueError("The number of blocks must be greater than the number of processes.")
+    d, Bd, sd = _batched(to_dev(diag), 3)
     if N > 1:
         u, Bu, su = _batched(to_dev(upper), 3)
         l, Bl, sl = _batched(to_dev(lower), 3)
@@ -647,19 +680,36 @@ def zgemm_batched(A, B, *, opA="N", opB="N", alpha=1.0, beta=0.0, C0=None, shift
 
     precision "int8x" forms the complex128 product from `slices` int8 digit
     planes per operand on the int8 matrix cores (sinv_zgemm_int8x_batched)."""
+    if opA not in nat.OPS or opB not in nat.OPS:
+        raise ValueError(f"opA / opB must be one of {sorted(nat.OPS)}")
+    if precision not in ("fp64", "mixed", "int8x"):
+        raise ValueError("precision must be fp64, mixed or int8x")
+    sa, sb = _shape(A), _shape(B)
+    if len(sa) not in (2, 3) or len(sb) not in (2, 3):
+        raise ValueError(f"A and B must be (M, K) / (K, N) or batched, got {sa} and {sb}")
+    ba = sa[0] if len(sa) == 3 else 1
+    bb = sb[0] if len(sb) == 3 else 1
+    batch = max(ba, bb)
+    if ba not in (1, batch) or bb not in (1, batch):
+        raise ValueError(f"batch sizes of A ({ba}) and B ({bb}) must be equal or 1")
+    M, K = sa[-2:] if opA == "N" else sa[-2:][::-1]
+    K2, N = sb[-2:] if opB == "N" else sb[-2:][::-1]
+    if K != K2:
+        raise ValueError("inner dimensions differ")
+    if C0 is not None:
+        sc = _shape(C0)
+        if len(sc) not in (2, 3) or sc[-2:] != (M, N) or (len(sc) == 3 and sc[0] not in (1, batch)):
+            raise ValueError(f"C0 must be ({M}, {N}) or ({batch}, {M}, {N}), got {sc}")
+    if shift is not None and int(np.prod(_shape(shift))) != batch:
+        raise ValueError(f"shift needs one entry per batch entry ({batch})")
     A = to_dev(A)
     B = to_dev(B)
     if A.dim() == 2:
         A = A[None]
     if B.dim() == 2:
         B = B[None]
-    batch = max(A.shape[0], B.shape[0])
-    ar, ac = A.shape[-2:]
-    br, bc = B.shape[-2:]
-    M, K = (ar, ac) if opA == "N" else (ac, ar)
-    K2, N = (br, bc) if opB == "N" else (bc, br)
-    if K != K2:
-        raise ValueError("inner dimensions differ")
+    ac = A.shape[-1]
+    bc = B.shape[-1]
     sA = A[0].numel() if A.shape[0] > 1 else 0
     sB = B[0].numel() if B.shape[0] > 1 else 0
     C = torch.empty((batch, M, N), dtype=_c128, device=A.device)

@@ -7,6 +7,9 @@ import glob
 import os
 import re
 
+import numpy as np
+import pytest
+
 from tests.conftest import ROOT
 
 
@@ -154,3 +157,49 @@ def test_int8x_host_validation(built_lib):
     assert w64 > 0 and w8 >= w64 + 2 * 4 * 3 * 7 * 128 * 128
     d.precision = 3
     assert L.sinv_rgf_workspace_bytes(ctypes.byref(d)) == 0
+
+
+def test_inconsistent_shapes_raise_before_any_launch():
+    """Every batched entry point checks block counts, block sizes and batch
+    sizes on the host before anything reaches the device (a mismatch would
+    otherwise read past the end of a buffer).  Runs without a GPU: the checks
+    come before the device transfer."""
+    from sinv_amd import core
+    d = np.zeros((4, 3, 3), complex)
+    u = np.zeros((3, 3, 3), complex)
+    bad_u = np.zeros((2, 3, 3), complex)
+    bad_b = np.zeros((3, 3, 2), complex)
+    for fn, extra in ((core.rgf_batched, ()), (core.bcr_batched, ()),
+                      (core.pdiv_batched, (2,))):
+        for uu, ll in ((bad_u, u), (u, bad_u), (bad_b, u)):
+            with pytest.raises(ValueError):
+                fn(d, uu, ll, *extra)
+    d12 = np.zeros((12, 3, 3), complex)
+    u11 = np.zeros((11, 3, 3), complex)
+    with pytest.raises(ValueError):
+        core.psr_batched(d12, u11, np.zeros((10, 3, 3), complex), 2)
+    with pytest.raises(ValueError):
+        core.psr_batched(np.zeros((12, 3, 2), complex), u11, u11, 2)
+    # the adjoint: gradient blocks of the wrong count or size
+    for gd, gu, gl in ((np.zeros((3, 3, 3), complex), u, u), (d, bad_u, u), (d, u, bad_b)):
+        with pytest.raises(ValueError):
+            core.rgf_adjoint_batched(d, u, u, gd, gu, gl)
+    with pytest.raises(ValueError):
+        core.rgf_adjoint_ragged([np.zeros((3, 3))] * 2, [np.zeros((3, 3))], [np.zeros((3, 3))],
+                                [np.zeros((2, 2))] * 2, [np.zeros((2, 2))], [np.zeros((2, 2))])
+    # GEMM: batches must be equal or 1; C0 and shift must match
+    A = np.zeros((2, 4, 5), complex)
+    B = np.zeros((3, 5, 6), complex)
+    with pytest.raises(ValueError, match="batch sizes"):
+        core.zgemm_batched(A, B)
+    B2 = np.zeros((2, 5, 6), complex)
+    with pytest.raises(ValueError, match="C0"):
+        core.zgemm_batched(A, B2, C0=np.zeros((2, 4, 7), complex))
+    with pytest.raises(ValueError, match="C0"):
+        core.zgemm_batched(A, B2, C0=np.zeros((3, 4, 6), complex))
+    with pytest.raises(ValueError, match="shift"):
+        core.zgemm_batched(A, B2, shift=np.zeros(3, complex))
+    with pytest.raises(ValueError, match="inner"):
+        core.zgemm_batched(A, np.zeros((2, 4, 6), complex))
+    with pytest.raises(ValueError):
+        core.zgemm_batched(A, B2, opA="X")

@@ -59,6 +59,29 @@ def test_bcr_block_counts(built_lib, N):
     assert dense.rel_err(got, ref) < 1e-13
 
 
+@pytest.mark.parametrize("N,b", [(7, 8), (16, 12), (21, 64), (33, 96)])
+def test_bcr_fill_blocks_are_blocks_of_the_inverse(built_lib, N, b):
+    """The stride-2^l blocks bcr_serial returns besides the tridiagonal ones
+    (fill-in of the reduction, bcr_serial.py:137-169, 308-383) against the
+    same blocks of np.linalg.inv, and the tridiagonal bands too, at
+    1e-15 cond(A) n."""
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    d, u, l = configs.dominant_blocks(N, b, seed=N * b)
+    Gd, Gu, Gl, Gf = core.bcr_batched(d, u, l, return_fill=True)
+    A = dense.assemble(d, u, l)
+    inv = np.linalg.inv(A)
+    tol = 1e-15 * np.linalg.cond(A) * A.shape[0] * np.abs(inv).max()
+    pos = core.bcr_fill_positions(N)
+    Gf = Gf[0].cpu().numpy()
+    assert Gf.shape[0] == len(pos)
+    for k, (r, c) in enumerate(pos):
+        assert np.abs(Gf[k] - inv[r * b:(r + 1) * b, c * b:(c + 1) * b]).max() <= tol, (r, c)
+    sel = dense.select(inv, b)
+    for g, ref in zip((Gd, Gu, Gl), sel):
+        assert np.abs(g[0].cpu().numpy() - np.stack(ref)).max() <= tol
+
+
 def test_bcr_dominant_16x64(built_lib):
     from sinv_amd import core
     from sinv_amd.utils import configs

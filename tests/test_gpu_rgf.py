@@ -163,3 +163,31 @@ def test_planset_concurrent_matches_sequential_and_replays(built_lib):
         g.replay()
         torch.cuda.synchronize()
         check_all(f"replay {rep}")
+
+
+@pytest.mark.parametrize("b", [128, 384])
+def test_rgf_headline_configuration_fp64(built_lib, b):
+    """The batched-energies workload's configuration: 64 blocks of b, A(E) =
+    (E + 1e-3 i) I - H with H from the workload's Hermitian generator, three
+    energies inside the band [-2, 2], FP64.  Each instance's selected blocks
+    against the oracle (rgf.py:40-62 restated, numpy inv / @), relative
+    Frobenius over all diag/upper/lower blocks.  Bound: max(1e-12, 4 x the
+    instance's rounding floor), the floor being the distance between the
+    oracle and the same recurrences with every triple product re-associated
+    (an equally valid FP64 evaluation order); the test prints both."""
+    from oracle import rgf_ref
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    eta = 1e-3
+    H = configs.hermitian_H(64, b, seed=10)
+    E = np.array([-1.3, 0.05, 0.9])
+    Gd, Gu, Gl = core.rgf_energies(*H, E, eta)
+    for k, e in enumerate(E):
+        d, u, l = configs.shifted(H, e + 1j * eta)
+        ref = rgf_ref.rgf_blocks(d, u, l)
+        alt = rgf_ref.rgf_blocks(d, u, l, reassociate=True)
+        got = (Gd[k].cpu().numpy(), Gu[k].cpu().numpy(), Gl[k].cpu().numpy())
+        err = dense.rel_err(got, ref)
+        floor = dense.rel_err(alt, ref)
+        print(f"b={b} E={e}: GPU vs oracle {err:.3e}, rounding floor {floor:.3e}", flush=True)
+        assert err <= max(1e-12, 4 * floor), (e, err, floor)

@@ -44,6 +44,48 @@ def test_compressed_schedule_is_bitwise_the_dense_one(n, b, cplx):
     assert not np.any(G[~mask])
 
 
+@pytest.mark.parametrize("n,b", BCR_SHAPES)
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("sym", [False, True])
+def test_fill_in_blocks_are_blocks_of_the_inverse(n, b, cplx, sym):
+    """bcr_serial returns the stride-2^l coupling blocks the reduction creates
+    (bcr_serial.py:137-169, 308-383); each is the same block of A^-1.  Closed
+    form, independent of any BCR restatement: every fill-in block of the
+    oracle's dense output against np.linalg.inv, at 1e-15 cond(A) n on the
+    diagonally dominant version of the reference's test matrix and at the
+    reference's own allclose on the raw one."""
+    from sinv_amd.core import bcr_fill_positions
+    N = n // b
+    raw = mu.generateBandedDiagonalMatrix(n, np.ceil(b / 2), cplx, sym, SEED)
+    for A, strict in ((raw + 3 * b * np.eye(n), True), (raw, False)):
+        G = bcr_ref.bcr_serial_dense(A, b)
+        inv = np.linalg.inv(A)
+        tol = 1e-15 * np.linalg.cond(A) * n
+        pos = bcr_fill_positions(N)
+        assert len(pos) == len(set(pos))
+        for r, c in pos:
+            got = G[r * b:(r + 1) * b, c * b:(c + 1) * b]
+            ref = inv[r * b:(r + 1) * b, c * b:(c + 1) * b]
+            if strict:
+                assert np.abs(got - ref).max() <= tol * np.abs(inv).max(), (r, c)
+            else:
+                assert np.allclose(got, ref), (r, c)
+
+
+def test_fill_in_pin_catches_a_wrong_block():
+    """The pin above is not vacuous: a fill-in block computed with the sign of
+    its update flipped (a plausible slip in center_produce) fails it."""
+    from sinv_amd.core import bcr_fill_positions
+    n, b = 144, 12
+    A = mu.generateBandedDiagonalMatrix(n, np.ceil(b / 2), True, False, SEED) + 3 * b * np.eye(n)
+    G = bcr_ref.bcr_serial_dense(A, b)
+    inv = np.linalg.inv(A)
+    r, c = bcr_fill_positions(n // b)[0]
+    blk = G[r * b:(r + 1) * b, c * b:(c + 1) * b]
+    assert np.abs(-blk - inv[r * b:(r + 1) * b, c * b:(c + 1) * b]).max() > 1e-6
+    assert np.abs(blk - inv[r * b:(r + 1) * b, c * b:(c + 1) * b]).max() < 1e-14
+
+
 def test_padding_helpers():
     # bcr_utils.py:35-40: 2^ceil(log2(N+1)) - 1 - N
     for N, pad in [(1, 0), (2, 1), (3, 0), (9, 6), (12, 3), (16, 15), (31, 0), (32, 31),

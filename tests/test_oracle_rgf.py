@@ -119,3 +119,19 @@ def test_planted_mistakes_are_caught(kind):
     G = _mistaken_rgf(kind, d, u, l)
     R = dense.selected_inverse_dense(d, u, l)
     assert dense.rel_err(G, R) > 1e-6
+
+
+def test_reassociated_order_is_an_equally_valid_evaluation():
+    """rgf_blocks(reassociate=True), used to measure an instance's rounding
+    floor, evaluates the same recurrences: it matches the dense inverse as
+    closely as the reference order does, and differs from it only by
+    rounding."""
+    from sinv_amd.utils import configs
+    H = configs.hermitian_H(12, 24, seed=4)
+    d, u, l = configs.shifted(H, 0.3 + 1e-3j)
+    ref = rgf_ref.rgf_blocks(d, u, l)
+    alt = rgf_ref.rgf_blocks(d, u, l, reassociate=True)
+    exact = dense.selected_inverse_dense(d, u, l)
+    e_ref, e_alt = dense.rel_err(ref, exact), dense.rel_err(alt, exact)
+    assert e_alt < 10 * max(e_ref, 1e-15) and e_ref < 1e-10
+    assert 0 < dense.rel_err(alt, ref) < 1e-10
