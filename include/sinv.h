@@ -103,6 +103,19 @@ typedef struct sinv_rgf_desc {
 size_t sinv_rgf_workspace_bytes(const sinv_rgf_desc* d);
 int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_bytes, void* stream);
 
+/* Allocation-free variant of sinv_rgf_batched.  No counterpart in the
+ * reference (it allocates through numpy); it is for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Same descriptor, arithmetic and results
+ * as sinv_rgf_batched; the workspace is a library-held device array of
+ * SINV_NOALLOC_SCRATCH_BYTES on each GPU (held until the process ends), so a
+ * call whose sinv_rgf_workspace_bytes exceeds that returns SINV_E_INVAL with
+ * the sizes in sinv_last_error().  Nothing is allocated.  Every call on one
+ * GPU shares that GPU's array: calls that overlap on a GPU (other streams or
+ * threads) race and must be serialised by the caller. */
+#define SINV_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int sinv_rgf_batched_noalloc(const sinv_rgf_desc* d, void* stream);
+
 /* ------------------------------------------------------------------------
  * Adjoint of the RGF selected inversion, batched over instances.
  * No counterpart in the reference (it has no gradients); it differentiates

@@ -125,6 +125,8 @@ def lib():
     L.sinv_rgf_workspace_bytes.argtypes = [ctypes.POINTER(RgfDesc)]
     L.sinv_rgf_batched.restype = _i
     L.sinv_rgf_batched.argtypes = [ctypes.POINTER(RgfDesc), _vp, ctypes.c_size_t, _vp]
+    L.sinv_rgf_batched_noalloc.restype = _i
+    L.sinv_rgf_batched_noalloc.argtypes = [ctypes.POINTER(RgfDesc), _vp]
     L.sinv_zinv_workspace_bytes.restype = ctypes.c_size_t
     L.sinv_zinv_workspace_bytes.argtypes = [_i, _i]
     L.sinv_zinv_batched.restype = _i

@@ -15,6 +15,7 @@
 // Schur GEMM's epilogue, and oscale folds into the GEMM alphas.
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "sinv_core.hpp"
@@ -238,4 +239,60 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
     if (Gl) SINV_HIP_TRY(launch_fill(Gl, L.osize * nb, nan, s));
   }
   return SINV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free RGF: the workspace is the file-scope device array below,
+// one copy per GPU (loaded with the code object, held until the process
+// ends).  Host code never passes the array's name to a kernel or a copy: its
+// device address is looked up once per device with hipGetSymbolAddress (with
+// that device current) and only that pointer is used.  Every call on a GPU
+// shares that GPU's array; nothing in the RGF path reads workspace it has not
+// written in the same call, so no clearing is needed, and calls that overlap
+// on one GPU race.
+static __device__ __align__(256) unsigned char scratch[SINV_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+std::mutex g_scratch_mu;
+
+hipError_t scratch_ptr(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" int sinv_rgf_batched_noalloc(const sinv_rgf_desc* d, void* stream) {
+  if (!desc_valid(d)) {
+    set_error("sinv_rgf_batched_noalloc: invalid descriptor");
+    return SINV_E_INVAL;
+  }
+  const size_t need = sinv_rgf_workspace_bytes(d);
+  if (need == 0) {
+    set_error("sinv_rgf_batched_noalloc: invalid block sizes");
+    return SINV_E_INVAL;
+  }
+  if (need > (size_t)SINV_NOALLOC_SCRATCH_BYTES) {
+    set_error("sinv_rgf_batched_noalloc: this call needs %zu bytes of workspace, more than the "
+              "%zu the library holds per GPU; split the batch or use sinv_rgf_batched",
+              need, (size_t)SINV_NOALLOC_SCRATCH_BYTES);
+    return SINV_E_INVAL;
+  }
+  void* p = nullptr;
+  SINV_HIP_TRY(scratch_ptr(&p));
+  return sinv_rgf_batched(d, p, (size_t)SINV_NOALLOC_SCRATCH_BYTES, stream);
 }
