@@ -56,6 +56,8 @@ def parse():
                              "gemm"])
     ap.add_argument("--chunk", type=int, default=64, help="energies per adjoint launch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the forward+adjoint record of the default run")
     ap.add_argument("--no-plain", action="store_true", help="skip the batch=1 FP64 sample")
     ap.add_argument("--no-bcr", action="store_true",
                     help="skip BCR-serial on the single-instance workloads")
@@ -560,21 +562,32 @@ def main():
         "kernel_time": {k: dict(ms=round(v[0], 3), launches=v[1], tflops=round(v[2] / (v[0] / 1e3) / 1e12, 3)
                                 if v[0] > 0 else 0.0) for k, v in results[head]["prof"].items() if v[1]},
     }
+    if args.workload == "energies" and not args.no_adjoint:
+        # the forward+adjoint workload (FP64, at most two timed steps after one
+        # warm-up, so the default run stays within minutes) as a second record
+        log("[bench] forward+adjoint workload (second record)")
+        out["forward_adjoint"] = adjoint_record(args, modes=("fp64",), steps=min(args.steps, 2),
+                                                warmup=min(args.warmup, 1),
+                                                cpu_baseline=not args.no_cpu)
     print(json.dumps(out), flush=True)
     return 0
 
 
-def bench_adjoint(args):
+def adjoint_record(args, modes=("fp64", "mixed", "int8x"), steps=None, warmup=None,
+                   cpu_baseline=True):
     """Batched forward + adjoint with checkpointing: 512 energies in [-2, 2],
     eta = 1e-3, one Hermitian H of 96 blocks whose sizes mix 256 and 416
     inside the instance; loss L = -(1/pi) sum_E sum_i Im tr G_ii(E); the
     gradient dL/dH on the block-tridiagonal pattern, summed over energies.
     A step is the loss and its full gradient over all energies, processed in
-    energy chunks (left-connected blocks checkpointed every 8 blocks)."""
+    energy chunks (left-connected blocks checkpointed every 8 blocks).
+    Returns the record (a dict); modes other than fp64 are compared with it."""
     import torch
     from sinv_amd import core, _native
     from sinv_amd.utils import configs
 
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     rng = np.random.default_rng(31)
     bs = [int(x) for x in rng.choice([256, 416], size=96)]
     H = configs.hermitian_H_ragged(bs, seed=32)
@@ -607,7 +620,7 @@ def bench_adjoint(args):
 
     out = {}
     tuned = {}
-    for prec in ("fp64", "mixed", "int8x"):
+    for prec in modes:
         if not args.no_tune and prec != "int8x":
             # every chunk issues the same GEMM shapes: tune on one (untimed)
             from sinv_amd import autotune
@@ -616,35 +629,36 @@ def bench_adjoint(args):
             tuned[prec] = dict(shapes=len(table), seconds=round(time.perf_counter() - t0, 2))
             log(f"[bench] autotuned {len(table)} GEMM shapes ({prec}) in {tuned[prec]['seconds']} s")
         log(f"[bench] adjoint workload {prec}: warmup")
+        torch.cuda.synchronize()
+        base_mem = torch.cuda.memory_allocated()
         torch.cuda.reset_peak_memory_stats()
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             step(prec)
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        for _ in range(args.steps):
+        for _ in range(steps):
             loss, grad = step(prec)
         e1.record()
         torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / args.steps
-        peak = torch.cuda.max_memory_allocated() / 1e9
+        ms = e0.elapsed_time(e1) / steps
+        # this workload's own peak (memory already held when it started excluded)
+        peak = (torch.cuda.max_memory_allocated() - base_mem) / 1e9
         log(f"[bench] {prec}: {ms:.1f} ms/step, {nE / (ms / 1e3):.1f} solves/s (fwd+adj), "
             f"peak {peak:.1f} GB")
         # kernel-level figures from one energy chunk (same shapes as every chunk)
         with _native.LaunchProfiler(max_launches=100000) as prof:
             step(prec, limit=args.chunk)
             torch.cuda.synchronize()
-        out[prec] = dict(ms=ms, loss=float(loss), grad=grad, prof=prof.stats, peak_gb=peak)
+        out[prec] = dict(ms=ms, loss=float(loss), grad=grad, prof=prof.stats, bytes=prof.bytes,
+                         peak_gb=peak)
+
     def grad_err(prec):
         num = sum(float(((a - b).abs() ** 2).sum()) for a, b in zip(out[prec]["grad"], out["fp64"]["grad"]))
         den = sum(float((b.abs() ** 2).sum()) for b in out["fp64"]["grad"])
         return (float(np.sqrt(num / den)),
                 abs(out[prec]["loss"] - out["fp64"]["loss"]) / abs(out["fp64"]["loss"]))
-
-    gerr, lerr = grad_err("mixed")
-    gerr8, lerr8 = grad_err("int8x")
-    log(f"[bench] gradient error vs fp64: mixed {gerr:.3e}, int8x {gerr8:.3e}")
 
     # plain FP64: one energy per call (batch = 1), on a sample of energies
     nplain = 3
@@ -665,20 +679,79 @@ def bench_adjoint(args):
     plain = dict(solves_per_s=round(1e3 / plain_ms, 3), ms_per_solve=round(plain_ms, 3),
                  sample=f"{nplain} energies, one forward+adjoint call each (batch = 1)")
     log(f"[bench] plain fp64 batch=1 forward+adjoint: {plain_ms:.1f} ms/solve")
+
+    # CPU baseline: the reference has no adjoint, so its forward RGF (the
+    # numpy restatement of rgf.py, same inv / @ calls) on sampled energies
+    cpu = None
+    if cpu_baseline:
+        from oracle import rgf_ref
+        cores = len(os.sched_getaffinity(0))
+        try:
+            from threadpoolctl import threadpool_info
+            info = [i for i in threadpool_info() if i.get("user_api") == "blas"]
+            if info:
+                cores = int(info[0]["num_threads"])
+        except Exception:
+            pass
+        d, u, l = configs.shifted_lists(H, E[0])
+        rgf_ref.rgf_lists(d[:16], u[:15], l[:15])  # warm-up
+        runs = []
+        for k in range(3):
+            d, u, l = configs.shifted_lists(H, E[k])
+            t0 = time.perf_counter()
+            rgf_ref.rgf_lists(d, u, l)
+            runs.append(time.perf_counter() - t0)
+        t = float(np.median(runs))
+        cpu = dict(value=round(1.0 / t, 4), unit="solves/s", cores=cores, kind="port",
+                   sample=f"forward only (the reference has no adjoint): numpy restatement of "
+                          f"rgf.py on this instance at 3 of the {nE} energies, median "
+                          f"{t:.2f} s per solve, all BLAS threads")
+        log(f"[bench] cpu oracle (forward only): {t:.2f} s/solve")
+
     fwd_flops = configs.rgf_flops_ragged(bs) * nE
     head = "fp64"
     ms = out[head]["ms"]
     st = out[head]["prof"]
     # the profiled chunk is one of ceil(nE / chunk) identical chunks
     nchunks = -(-nE // args.chunk)
-    executed = sum(v[2] for v in st.values()) * nchunks
+    executed = sum(v[2] for v in st.values() if v[1]) * nchunks
     kms, kcnt, kfl = st["gemm_f64"]
     achieved = kfl / (kms / 1e3) / 1e12 if kms > 0 else 0.0
-    res = {
+    kname = "gemm_kernel<double>"
+    roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_TFLOPS["gemm_f64"],
+                unit="TFLOP/s", frac=round(achieved / PEAK_TFLOPS["gemm_f64"], 4),
+                traffic=measured_traffic("rgf_forward_adjoint", kname), kernel=kname,
+                launches=kcnt, avg_launch_us=round(kms / max(kcnt, 1) * 1e3, 3),
+                flops_per_launch=kfl / max(kcnt, 1),
+                algorithmic_bytes_per_launch=out[head]["bytes"]["gemm_f64"] / max(kcnt, 1),
+                mfma_executed_tflops=round(achieved * 0.75, 3),
+                mfma_executed_frac=round(achieved * 0.75 / PEAK_TFLOPS["gemm_f64"], 4),
+                timed="launch profiler over one energy chunk (hipEvents on the launch stream)",
+                traffic_source="profiles/traffic_rgf_forward_adjoint.txt (rocprofv3 --pmc "
+                               "FETCH_SIZE + WRITE_SIZE, calibrated; null if absent)")
+    if roof["traffic"] is not None and kcnt:
+        hbm = roof["traffic"] / (kms / kcnt / 1e3) / 1e9
+        roof["hbm_GBps"] = round(hbm, 1)
+        roof["hbm_frac"] = round(hbm / (HBM_TBPS * 1e3), 4)
+    mode_rec = {"fp64": dict(solves_per_s=round(nE / (out["fp64"]["ms"] / 1e3), 3),
+                             ms_per_step=round(out["fp64"]["ms"], 3),
+                             peak_hbm_gb=round(out["fp64"]["peak_gb"], 2))}
+    for prec in modes:
+        if prec == "fp64":
+            continue
+        gerr, lerr = grad_err(prec)
+        log(f"[bench] gradient error vs fp64: {prec} {gerr:.3e}")
+        mode_rec[prec] = dict(solves_per_s=round(nE / (out[prec]["ms"] / 1e3), 3),
+                              ms_per_step=round(out[prec]["ms"], 3),
+                              grad_rel_err_vs_fp64=gerr, loss_rel_err_vs_fp64=lerr,
+                              passes_err_gate=gerr <= ERR_GATE,
+                              peak_hbm_gb=round(out[prec]["peak_gb"], 2))
+    mode_rec["plain_fp64_batch1"] = plain
+    return {
         "metric": METRIC,
         "value": round(nE / (ms / 1e3), 3),
         "unit": "solves/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
         "config": dict(workload="rgf_forward_adjoint_dos", blocks=96, block_sizes="256/416 mixed",
@@ -689,32 +762,18 @@ def bench_adjoint(args):
         "forward_algorithmic_tflops": round(fwd_flops / (ms / 1e3) / 1e12, 3),
         "peak_hbm_gb": round(out[head]["peak_gb"], 2),
         "gemm_autotune": tuned or None,
-        "modes": {
-            "fp64": dict(solves_per_s=round(nE / (out["fp64"]["ms"] / 1e3), 3),
-                         ms_per_step=round(out["fp64"]["ms"], 3), peak_hbm_gb=round(out["fp64"]["peak_gb"], 2)),
-            "mixed": dict(solves_per_s=round(nE / (out["mixed"]["ms"] / 1e3), 3),
-                          ms_per_step=round(out["mixed"]["ms"], 3),
-                          grad_rel_err_vs_fp64=gerr, loss_rel_err_vs_fp64=lerr,
-                          passes_err_gate=gerr <= ERR_GATE),
-            "int8x": dict(solves_per_s=round(nE / (out["int8x"]["ms"] / 1e3), 3),
-                          ms_per_step=round(out["int8x"]["ms"], 3),
-                          grad_rel_err_vs_fp64=gerr8, loss_rel_err_vs_fp64=lerr8,
-                          passes_err_gate=gerr8 <= ERR_GATE,
-                          peak_hbm_gb=round(out["int8x"]["peak_gb"], 2)),
-            "plain_fp64_batch1": plain,
-        },
+        "modes": mode_rec,
         "speedup_vs_plain_fp64": {m: round(nE / (out[m]["ms"] / 1e3) / plain["solves_per_s"], 2)
-                                  for m in ("fp64", "mixed", "int8x")},
-        "roofline": dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_TFLOPS["gemm_f64"],
-                         unit="TFLOP/s", frac=round(achieved / PEAK_TFLOPS["gemm_f64"], 4),
-                         traffic=None, kernel="gemm_kernel<double>", launches=kcnt,
-                         avg_launch_us=round(kms / max(kcnt, 1) * 1e3, 3)),
-        "cpu_baseline": None,
+                                  for m in modes},
+        "roofline": roof,
+        "cpu_baseline": cpu,
         "kernel_time": {k: dict(ms=round(v[0], 3), launches=v[1]) for k, v in st.items() if v[1]},
     }
-    print(json.dumps(res), flush=True)
-    return 0
 
+
+def bench_adjoint(args):
+    print(json.dumps(adjoint_record(args, cpu_baseline=not args.no_cpu)), flush=True)
+    return 0
 
 
 def bench_gemm(args):
