@@ -64,3 +64,52 @@ def test_pdiv_rejects_bad_partition_counts(built_lib):
         core.pdiv_batched(d, u, l, 3)
     with pytest.raises(ValueError, match="greater than the number of processes"):
         core.pdiv_batched(d, u, l, 8)
+
+
+@pytest.mark.parametrize("n,b,P", [(9, 3, 2), (128, 8, 4), (128, 16, 8), (128, 32, 2), (6, 3, 2)])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_pdiv_localmap_per_rank_workflow(built_lib, n, b, P, cplx):
+    """The reference's per-rank workflow (pdiv_test.py:81-108): check, divide,
+    partition, then every rank calls pdiv_lm.pdiv_localmap(K_i[rank], Bu_i,
+    Bl_i, blocksize) -- here P ranks as threads of one process
+    (pdiv_u.LocalComm).  Each rank's lists match the oracle's per-rank output
+    and the reference's allclose against the dense inverse, with zeros in the
+    last rank's bridge slots."""
+    from sinv_amd.algorithms import pdiv_lm, pdiv_u
+    from sinv_amd.utils import matu
+    print(f"pdiv_localmap n={n} b={b} P={P} cplx={cplx}", flush=True)
+    A = matu.generateBandedDiagonalMatrix(n, np.ceil(b / 2), cplx, False, 63) + 3 * b * np.eye(n)
+    pdiv_u.check_multiprocessing(P)
+    pdiv_u.check_input(A, b, P)
+    starts, sizes = pdiv_u.divide_matrix(A, P, b)
+    K_i, Bu_i, Bl_i = pdiv_u.partition_subdomain(A, starts, sizes, b)
+    out = pdiv_u.LocalComm(P).run(
+        lambda comm: pdiv_lm.pdiv_localmap(K_i[comm.Get_rank()], Bu_i, Bl_i, b, comm=comm))
+    ref = pdiv_ref.pdiv_blocks(A, b, P)
+    R = np.linalg.inv(A)
+    nblocks = n // b
+    for rank, ((D, U, L), (rD, rU, rL)) in enumerate(zip(out, ref)):
+        assert len(D) == len(U) == len(L) == sizes[rank]
+        assert np.asarray(D).dtype == A.dtype
+        for g, r in ((D, rD), (U, rU), (L, rL)):
+            assert np.abs(np.stack(g) - np.stack(r)).max() <= 1e-12 * max(np.abs(r).max(), 1e-300)
+        for i in range(sizes[rank]):
+            ip = starts[rank] + i
+            assert np.allclose(D[i], R[ip * b:(ip + 1) * b, ip * b:(ip + 1) * b])
+            if ip < nblocks - 1:
+                assert np.allclose(U[i], R[ip * b:(ip + 1) * b, (ip + 1) * b:(ip + 2) * b])
+                assert np.allclose(L[i], R[(ip + 1) * b:(ip + 2) * b, ip * b:(ip + 1) * b])
+            else:
+                assert not np.any(U[i]) and not np.any(L[i])
+
+
+def test_pdiv_localmap_single_rank(built_lib):
+    """With no communicator (one rank, as a serial caller of the reference
+    has) the partition is the whole matrix and there are no bridges."""
+    from sinv_amd.algorithms import pdiv_lm
+    from sinv_amd.utils import matu
+    A = matu.generateBandedDiagonalMatrix(48, 4, True, False, 63) + 24 * np.eye(48)
+    D, U, L = pdiv_lm.pdiv_localmap(A, [], [], 8)
+    sel = dense.select(np.linalg.inv(A), 8)
+    assert np.abs(np.stack(D) - sel[0]).max() < 1e-14
+    assert np.abs(np.stack(U[:-1]) - sel[1]).max() < 1e-14 and not np.any(U[-1])

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import dense, pdiv_ref
-from sinv_amd.algorithms import pdiv as pdiv_dropin
+from sinv_amd.algorithms import pdiv_utils as pdiv_dropin
 from sinv_amd.utils import matu
 
 SHAPES = [(2, 1), (4, 2), (6, 3), (3, 1), (6, 2), (9, 3), (128, 8), (128, 16), (128, 32)]
@@ -94,3 +94,30 @@ def test_dropin_helpers_match_oracle_and_checks():
         pdiv_dropin.check_input(np.zeros((10, 10)), 3, 2)
     with pytest.raises(ValueError, match="greater than the number of processes"):
         pdiv_dropin.check_input(np.zeros((12, 12)), 4, 4)
+
+
+def test_local_comm_allgather_is_collective():
+    """pdiv_u.LocalComm runs P ranks as threads: allgather returns every
+    rank's object in rank order to every rank, twice in a row, and an error
+    on one rank is raised in the caller instead of hanging the others."""
+    world = pdiv_dropin.LocalComm(4)
+
+    def body(comm):
+        r = comm.Get_rank()
+        a = comm.allgather(r * 10)
+        b = comm.allgather(("x", r))
+        return comm.Get_size(), a, b
+
+    out = world.run(body)
+    assert all(o == (4, [0, 10, 20, 30], [("x", 0), ("x", 1), ("x", 2), ("x", 3)]) for o in out)
+
+    def fails(comm):
+        if comm.Get_rank() == 2:
+            raise RuntimeError("rank 2")
+        return comm.allgather(1)
+
+    with pytest.raises(RuntimeError, match="rank 2"):
+        pdiv_dropin.LocalComm(4, timeout=30).run(fails)
+    with pytest.raises(ValueError, match="power of 2"):
+        pdiv_dropin.LocalComm(3)
+    assert pdiv_dropin.SELF.allgather(5) == [5] and pdiv_dropin.SELF.Get_size() == 1
