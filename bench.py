@@ -231,7 +231,7 @@ def main():
 
     results = {}
     tuned = {}
-    for prec in ("fp64", "mixed", "int8x"):
+    for prec in ("fp64", "mixed", "mixed_refined", "int8x"):
         log(f"[bench] {wl['workload']}: batched {prec}: allocating plans")
         plans = make_plans(prec)
         if prec == "int8x":
@@ -266,8 +266,9 @@ def main():
         torch.cuda.synchronize()
 
     # ------------------------------------------------------------ error of mixed vs FP64, all instances
-    log("[bench] error of mixed and int8x vs fp64 over every instance")
+    log("[bench] error of mixed, mixed_refined and int8x vs fp64 over every instance")
     err_mixed = 0.0
+    err_ref = 0.0
     err_i8x = 0.0
     chunk = 16
     for gi, g in enumerate(groups):
@@ -276,16 +277,18 @@ def main():
             sl = slice(s0, min(nE, s0 + chunk))
             sh = None if g["shift"] is None else g["shift"][sl]
             outs = []
-            for prec in ("fp64", "mixed", "int8x"):
+            for prec in ("fp64", "mixed", "mixed_refined", "int8x"):
                 p = core.RgfPlan(*g["dev"], shift=sh, dscale=g["dscale"], oscale=g["oscale"],
                                  precision=prec, batch=None)
                 p.run()
                 outs.append(p.outputs())
             err_mixed = max(err_mixed, rel_err_per_instance(outs[1], outs[0]))
-            err_i8x = max(err_i8x, rel_err_per_instance(outs[2], outs[0]))
+            err_ref = max(err_ref, rel_err_per_instance(outs[2], outs[0]))
+            err_i8x = max(err_i8x, rel_err_per_instance(outs[3], outs[0]))
             del outs, p
     torch.cuda.synchronize()
-    log(f"[bench] mixed max rel err vs fp64: {err_mixed:.3e}; int8x: {err_i8x:.3e}")
+    log(f"[bench] max rel err vs fp64: mixed {err_mixed:.3e}, mixed_refined {err_ref:.3e}, "
+        f"int8x {err_i8x:.3e}")
 
     # ------------------------------------------------------------ FP64 vs the CPU oracle (sample)
     err_fp64 = None
@@ -387,12 +390,12 @@ def main():
     # ------------------------------------------------------------ pick the headline
     fp64_sps = solves / (results["fp64"]["ms"] / 1e3)
     mixed_sps = solves / (results["mixed"]["ms"] / 1e3)
+    ref_sps = solves / (results["mixed_refined"]["ms"] / 1e3)
     i8x_sps = solves / (results["int8x"]["ms"] / 1e3)
-    head = "fp64"
-    if err_mixed <= ERR_GATE and mixed_sps > fp64_sps:
-        head = "mixed"
-    if err_i8x <= ERR_GATE and i8x_sps > max(fp64_sps, mixed_sps if head == "mixed" else 0.0):
-        head = "int8x"
+    # the headline is the fastest mode inside the error gate (FP64 always is)
+    cands = {"fp64": (fp64_sps, 0.0), "mixed": (mixed_sps, err_mixed),
+             "mixed_refined": (ref_sps, err_ref), "int8x": (i8x_sps, err_i8x)}
+    head = max((m for m, (_, e) in cands.items() if e <= ERR_GATE), key=lambda m: cands[m][0])
     ms = results[head]["ms"]
 
     # ------------------------------------------------------------ PCIe-inclusive rate (never `value`)
@@ -497,10 +500,11 @@ def main():
                                  f"complex arrays ({4 * npad * npad * 16 / 1e9:.0f} GB)")
         log(f"[bench] bcr-serial: {bcr}")
 
-    kind = {"fp64": "gemm_f64", "mixed": "gemm_f32", "int8x": "gemm_i8x"}[head]
+    kind = {"fp64": "gemm_f64", "mixed": "gemm_f32", "mixed_refined": "gemm_f64",
+            "int8x": "gemm_i8x"}[head]
     kms, kcnt, kfl = results[head]["prof"][kind]
     kname = {"fp64": "gemm_kernel<double>", "mixed": "gemm_kernel<float>",
-             "int8x": "gemm_i8x_kernel<7>"}[head]
+             "mixed_refined": "gemm_kernel<double>", "int8x": "gemm_i8x_kernel<7>"}[head]
     achieved = kfl / (kms / 1e3) / 1e12 if kms > 0 else 0.0
     if head == "int8x":
         # executed int8 operations: 3 real products x 28 digit pairs x 2 MNK = 21 x 8 MNK
@@ -536,6 +540,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "complex128", "mixed": "complex64-gemm/complex128-inverse",
+                  "mixed_refined": "complex64-schur-gemm/complex128-refinement",
                   "int8x": "int8-digit-gemm/complex128"}[head],
         "data": "synthetic",
         "config": wl,
@@ -546,6 +551,11 @@ def main():
                                  err_vs_cpu_oracle=err_fp64),
             "batched_mixed": dict(solves_per_s=round(mixed_sps, 3), ms_per_step=round(results["mixed"]["ms"], 3),
                                   max_rel_err_vs_fp64=err_mixed, passes_err_gate=err_mixed <= ERR_GATE),
+            "batched_mixed_refined": dict(
+                solves_per_s=round(ref_sps, 3), ms_per_step=round(results["mixed_refined"]["ms"], 3),
+                max_rel_err_vs_fp64=err_ref, passes_err_gate=err_ref <= ERR_GATE,
+                method="complex64 forward Schur GEMMs, FP64 inverses, one FP64 refinement step "
+                       "of each g_i against M_i formed in FP64, FP64 backward sweep"),
             "batched_int8x": dict(solves_per_s=round(i8x_sps, 3), ms_per_step=round(results["int8x"]["ms"], 3),
                                   max_rel_err_vs_fp64=err_i8x, passes_err_gate=err_i8x <= ERR_GATE,
                                   digit_planes=7,

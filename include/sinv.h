@@ -46,6 +46,10 @@ extern "C" {
                           /* per real product), FP64 inverses; taken by the  */
                           /* RGF and adjoint drivers; sinv_zgemm_batched     */
                           /* rejects it (sinv_zgemm_int8x_batched instead)   */
+#define SINV_PREC_MIXED_REFINED 3 /* RGF only: complex64 forward Schur GEMMs, FP64 */
+                          /* inverses, one FP64 refinement step of each g_i  */
+                          /* against M_i formed in FP64 (R = M_i g_i - I,     */
+                          /* g_i -= g_i R), FP64 backward sweep              */
 
 #define SINV_OP_N 0
 #define SINV_OP_T 1

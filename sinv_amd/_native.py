@@ -16,7 +16,7 @@ SINV_E_INVAL = 2
 SINV_E_HIP = 3
 SINV_INFO_OK = (1 << 64) - 1
 
-PREC = {"fp64": 0, "mixed": 1, "int8x": 2}
+PREC = {"fp64": 0, "mixed": 1, "int8x": 2, "mixed_refined": 3}
 OPS = {"N": 0, "T": 1, "C": 2}
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsinv_hip.so")

@@ -8,6 +8,10 @@
 //                             G_{i,i+1} = G_{i+1,i}^T (sym) | -g_i A_{i,i+1} G_{i+1,i+1}
 //                             G_ii      = g_i + g_i A_{i,i+1} X_i
 //
+// Precision SINV_PREC_MIXED_REFINED runs the forward Schur products in
+// complex64 and refines each g~_i = inv(M~_i) once in FP64 against M_i formed
+// in FP64: R = M_i g~_i - I, g_i = g~_i - g~_i R; the backward sweep is FP64.
+//
 // Every step is one batched launch over all instances (energies).  The
 // product g_i A_{i,i+1} is formed once and shared by G_{i,i+1} and G_ii (the
 // reference forms it twice), and G_ii is accumulated in place over g_i.  A(E)
@@ -58,7 +62,8 @@ WorkPlan plan_work(const Layout& L, int batch, int precision) {
     g.outer = batch;
     w.i8x_bytes = gemm_i8x_workspace_bytes(g, kI8xDefaultSlices);
   }
-  w.total_bytes = (w.tmp_elems + w.inv_elems) * sizeof(cplx) + w.i8x_bytes + 256;
+  // -1 per instance: the shift of the refinement residual M g~ - I
+  w.total_bytes = (w.tmp_elems + w.inv_elems + (size_t)batch) * sizeof(cplx) + w.i8x_bytes + 256;
   return w;
 }
 
@@ -67,7 +72,7 @@ bool desc_valid(const sinv_rgf_desc* d) {
   if (d->nblocks <= 0 || d->batch <= 0) return false;
   if (d->bsizes == nullptr && d->blocksize <= 0) return false;
   if (d->precision != SINV_PREC_FP64 && d->precision != SINV_PREC_MIXED &&
-      d->precision != SINV_PREC_INT8X)
+      d->precision != SINV_PREC_INT8X && d->precision != SINV_PREC_MIXED_REFINED)
     return false;
   if (d->diag == nullptr || d->g_diag == nullptr) return false;
   if (d->nblocks > 1 && (d->upper == nullptr || d->lower == nullptr)) return false;
@@ -121,7 +126,10 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
   }
   hipStream_t s = (hipStream_t)stream;
   const int N = L.N, nb = d->batch;
-  const Prec prec = d->precision == SINV_PREC_MIXED ? Prec::F32 : Prec::F64;
+  const bool refine = d->precision == SINV_PREC_MIXED_REFINED;
+  // forward Schur products; the backward sweep is FP64 when refining
+  const Prec prec = d->precision == SINV_PREC_MIXED || refine ? Prec::F32 : Prec::F64;
+  const Prec bprec = refine ? Prec::F64 : prec;
   const bool i8x = d->precision == SINV_PREC_INT8X;
   const long long b2 = (long long)L.bmax * L.bmax;
 
@@ -143,7 +151,8 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
   cplx* Q = P + b2 * nb;
   cplx* Xw = Q + b2 * nb;
   cplx* winv = Xw + b2 * nb;
-  void* wi8x = winv + wp.inv_elems;
+  cplx* mones = winv + wp.inv_elems;  // nb entries of -1 (refinement only)
+  void* wi8x = mones + nb;
 
   // one GEMM launch over all instances: instance z is the outer batch index
   auto gemm = [&](int M, int Nn, int K, const cplx* A, int lda, long long sA, const cplx* B,
@@ -163,6 +172,7 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
   };
 
   if (d->info != nullptr) SINV_HIP_TRY(launch_fill_u64(d->info, 1, SINV_INFO_OK, s));
+  if (refine) SINV_HIP_TRY(launch_fill(mones, nb, cplx{-1.0, 0.0}, s));
 
   // ---- forward sweep (rgf.py:44-48).  Blocks above the leaf size form M_i in
   // the temporary Q and invert it into the g_i slot (the elimination works in
@@ -200,6 +210,23 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
     SINV_HIP_TRY(gemm(bi, bi, bp, P, bp, b2, U + L.ooff[i - 1], bi, d->upper_stride, dst, bi, sdst,
                       mosq, D + L.doff[i], bi, d->diag_stride, dscale, shift, prec));
     SINV_HIP_TRY(invert_m(i));
+    if (refine) {
+      // one FP64 refinement step of g~_i against M_i formed in FP64 (in Q,
+      // free once the inverse has consumed it)
+      cplx* gi = Gd + L.doff[i];
+      SINV_HIP_TRY(gemm(bi, bp, bp, Lo + L.ooff[i - 1], bp, d->lower_stride, Gd + L.doff[i - 1],
+                        bp, L.dsize, P, bp, b2, one, nullptr, 0, 0, zero, nullptr, Prec::F64));
+      SINV_HIP_TRY(gemm(bi, bi, bp, P, bp, b2, U + L.ooff[i - 1], bi, d->upper_stride, Q, bi, b2,
+                        mosq, D + L.doff[i], bi, d->diag_stride, dscale, shift, Prec::F64));
+      // R = M_i g~_i - I                     (in Xw)
+      SINV_HIP_TRY(gemm(bi, bi, bi, Q, bi, b2, gi, bi, L.dsize, Xw, bi, b2, one, nullptr, 0, 0,
+                        zero, mones, Prec::F64));
+      // g_i = g~_i - g~_i R                  (in Q, then back to the g_i slot)
+      SINV_HIP_TRY(gemm(bi, bi, bi, gi, bi, L.dsize, Xw, bi, b2, Q, bi, b2, mone, gi, bi, L.dsize,
+                        one, nullptr, Prec::F64));
+      SINV_HIP_TRY(launch_axpby_shift(bi, bi, one, Q, bi, 0, b2, gi, bi, 0, L.dsize, nullptr, 1,
+                                      nb, s));
+    }
   }
 
   // ---- backward sweep (rgf.py:51-62)
@@ -210,13 +237,13 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
     const long long sX = save ? L.osize : b2;
     // P = oscale G_{i+1,i+1} lower[i]     (b_{i+1} x b_i)
     SINV_HIP_TRY(gemm(bj, bi, bj, Gd + L.doff[i + 1], bj, L.dsize, Lo + L.ooff[i], bi,
-                      d->lower_stride, P, bi, b2, oscale, nullptr, 0, 0, zero, nullptr, prec));
+                      d->lower_stride, P, bi, b2, oscale, nullptr, 0, 0, zero, nullptr, bprec));
     // G_{i+1,i} = -P g_i
     SINV_HIP_TRY(gemm(bj, bi, bi, P, bi, b2, Gd + L.doff[i], bi, L.dsize, Xn, bi, sX, mone,
-                      nullptr, 0, 0, zero, nullptr, prec));
+                      nullptr, 0, 0, zero, nullptr, bprec));
     // Q = oscale g_i upper[i]              (b_i x b_{i+1})
     SINV_HIP_TRY(gemm(bi, bj, bi, Gd + L.doff[i], bi, L.dsize, U + L.ooff[i], bj,
-                      d->upper_stride, Q, bj, b2, oscale, nullptr, 0, 0, zero, nullptr, prec));
+                      d->upper_stride, Q, bj, b2, oscale, nullptr, 0, 0, zero, nullptr, bprec));
     if (save) {
       if (d->sym_mat) {
         SINV_HIP_TRY(launch_transpose(bi, bj, OP_T, Xn, bi, 0, sX, Gu + L.ooff[i], bj, 0,
@@ -225,12 +252,12 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
         // G_{i,i+1} = -Q G_{i+1,i+1}
         SINV_HIP_TRY(gemm(bi, bj, bj, Q, bj, b2, Gd + L.doff[i + 1], bj, L.dsize,
                           Gu + L.ooff[i], bj, L.osize, mone, nullptr, 0, 0, zero, nullptr,
-                          prec));
+                          bprec));
       }
     }
     // G_ii = g_i + Q X_i = g_i - Q G_{i+1,i}   (in place over g_i)
     SINV_HIP_TRY(gemm(bi, bi, bj, Q, bj, b2, Xn, bi, sX, Gd + L.doff[i], bi, L.dsize, mone,
-                      Gd + L.doff[i], bi, L.dsize, one, nullptr, prec));
+                      Gd + L.doff[i], bi, L.dsize, one, nullptr, bprec));
   }
   if (!save && N > 1) {
     // rgf.py:41,55: unsaved off-diagonal blocks of G stay NaN
