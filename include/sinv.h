@@ -197,6 +197,17 @@ typedef struct sinv_bcr_desc {
 size_t sinv_bcr_workspace_bytes(const sinv_bcr_desc* d);
 long long sinv_bcr_fill_blocks(int nblocks);
 int sinv_bcr_batched(const sinv_bcr_desc* d, void* work, size_t work_bytes, void* stream);
+/* BCR-parallel's per-rank work on one GPU (bcr_parallel.py:21-78): the same
+ * reduction and production, with every level operation issued as one launch
+ * per partition over the rows that partition owns -- npart partitions of the
+ * padded block rows split as bcr_utils.py:133-184 does among ranks (equal
+ * strides, the last takes the rest); a partition reduces its kept rows
+ * (bcr_parallel.py:227-256) and produces its rows (:1004-1015).  Each
+ * operation family is issued for all partitions before the next, which is
+ * where the ranks exchange halo rows.  Results equal sinv_bcr_batched's bit
+ * for bit; same workspace.  SINV_E_INVAL unless 1 <= npart <= padded rows. */
+int sinv_bcr_parallel_batched(const sinv_bcr_desc* d, int npart, void* work, size_t work_bytes,
+                              void* stream);
 
 /* ------------------------------------------------------------------------
  * PSR selected inversion on one GPU, batched over instances.

@@ -187,6 +187,9 @@ def lib():
         L.sinv_bcr_fill_blocks.argtypes = [_i]
         L.sinv_bcr_batched.restype = _i
         L.sinv_bcr_batched.argtypes = [ctypes.POINTER(BcrDesc), _vp, ctypes.c_size_t, _vp]
+        L.sinv_bcr_parallel_batched.restype = _i
+        L.sinv_bcr_parallel_batched.argtypes = [ctypes.POINTER(BcrDesc), _i, _vp, ctypes.c_size_t,
+                                                _vp]
     if hasattr(L, "sinv_psr_batched"):
         L.sinv_psr_workspace_bytes.restype = ctypes.c_size_t
         L.sinv_psr_workspace_bytes.argtypes = [ctypes.POINTER(PsrDesc)]

@@ -9,18 +9,24 @@ halo rows with its neighbours between levels and gathering each rank's rows
 of G on rank 0 (agregate_result_on_root :1019-1060).  Its arithmetic is
 block cyclic reduction's, the same as bcr_serial's.
 
-On one device the partitions play the part of the ranks: ``reduction_schedule``
-and ``production_schedule`` give, level by level, the rows each partition
-handles, and since they cover every row of a level exactly once (checked on
-every call), one level-batched launch over all partitions' rows (the device
-BCR, ``sinv_bcr_batched``) does exactly the ranks' work of that level, with
-the halo exchange replaced by the rows staying resident in HBM.  The call
-returns rank 0's gathered result: the dense matrix with the selected
-inverse's tridiagonal blocks and the fill-in blocks of the cyclic
-reduction, cropped to (n // blocksize) * blocksize.  Which extra off-band
-blocks a rank holds when the reference gathers is a detail of its message
-pattern, not checked by its test (bcr_parallel_test.py:70-79 compares the
-tridiagonal blocks); here they are the fill-in blocks of bcr_serial.
+On one device the partitions play the part of the ranks: the device BCR
+runs with ``n_partitions`` (``sinv_bcr_parallel_batched``), issuing every
+level operation as one launch per partition over the rows the partition owns
+-- the rank split of bcr_utils.divide_matrix on the padded rows, reduction of
+its kept rows and production of its rows as the ranks do -- and each
+operation family for all partitions before the next, which is where the
+ranks exchange halo rows (here the rows stay resident in HBM).
+``reduction_schedule`` and ``production_schedule`` restate the reference's
+per-level, per-rank row ranges; they cover every row of a level exactly once
+(checked on every call), and the device's per-partition launches follow the
+same ownership (tests compare the two).  The call returns rank 0's gathered
+result: the dense matrix with the selected inverse's tridiagonal blocks and
+the fill-in blocks of the cyclic reduction, cropped to (n // blocksize) *
+blocksize; the arithmetic equals bcr_serial's bit for bit.  Which extra
+off-band blocks a rank holds when the reference gathers is a detail of its
+message pattern, not checked by its test (bcr_parallel_test.py:70-79
+compares the tridiagonal blocks); here they are the fill-in blocks of
+bcr_serial.
 """
 
 import math
@@ -111,4 +117,4 @@ def bcr_parallel(A, blocksize, n_partitions=1):
     if n_partitions > nbp:
         raise ValueError("more partitions than padded block rows")
     check_schedule_covers(nbp, n_partitions)
-    return _serial.bcr_serial(A, blocksize)
+    return _serial.bcr_serial(A, blocksize, _npart=n_partitions)

@@ -30,30 +30,30 @@ def _blocks_of(A, b, N):
     return d, u, l
 
 
-def _bcr_serial_device(A, b):
+def _bcr_serial_device(A, b, npart=1):
     # A resident in HBM (torch device tensor): gather, solve and assemble the
     # dense result on the device; returns a device tensor
     import torch
     N = A.shape[0] // b
     Ad = A[:N * b, :N * b].to(torch.complex128)
     d, u, l = core.dense_to_blocks(Ad, b)
-    Gd, Gu, Gl, Gf = core.bcr_batched(d, u, l, return_fill=True)
+    Gd, Gu, Gl, Gf = core.bcr_batched(d, u, l, return_fill=True, npart=npart)
     G = core.blocks_to_dense(Gd, Gu, Gl)
     core.scatter_blocks(G, Gf, core.bcr_fill_positions(N))
     G = G[0]
     return G.real.to(A.dtype) if not A.is_complex() else G.to(A.dtype)
 
 
-def bcr_serial(A, blocksize):
+def bcr_serial(A, blocksize, _npart=1):
     from ..utils.matu import _on_device
     if _on_device(A):
-        return _bcr_serial_device(A, int(blocksize))
+        return _bcr_serial_device(A, int(blocksize), _npart)
     A = np.asarray(A)
     b = int(blocksize)
     N = A.shape[0] // b
     real = not np.iscomplexobj(A)
     d, u, l = _blocks_of(A, b, N)
-    Gd, Gu, Gl, Gf = core.bcr_batched(d, u, l, return_fill=True)
+    Gd, Gu, Gl, Gf = core.bcr_batched(d, u, l, return_fill=True, npart=_npart)
     Gd, Gu, Gl, Gf = (x[0].cpu().numpy() for x in (Gd, Gu, Gl, Gf))
     G = np.zeros((N * b, N * b), dtype=np.complex128)
     for i in range(N):

@@ -261,12 +261,15 @@ class PlanSet:
 
 
 def bcr_batched(diag, upper, lower, *, shift=None, dscale=1.0, oscale=1.0, return_fill=False,
-                check=True):
+                check=True, npart=1):
     """BCR-serial selected inversion of a batch of instances (bcr_serial.py:19-466).
 
     Same instance convention as ``rgf_batched`` (uniform block size).  Returns
     (G_diag, G_upper, G_lower) and, with ``return_fill``, the produced blocks
     of levels >= 1 as (B, F, b, b) in the order documented in include/sinv.h.
+    ``npart > 1`` issues every level operation per partition of the padded
+    block rows, as BCR-parallel's ranks split them (sinv_bcr_parallel_batched;
+    same bits).
     """
     _check_tridiag(diag, upper, lower)
     d, Bd, sd = _batched(to_dev(diag), 3)
@@ -294,7 +297,11 @@ def bcr_batched(diag, upper, lower, *, shift=None, dscale=1.0, oscale=1.0, retur
         shift=_p(sh), g_diag=_p(Gd), g_upper=_p(Gu), g_lower=_p(Gl), g_fill=_p(Gf),
         info=_info_ptr(info))
     ws = _workspace(L.sinv_bcr_workspace_bytes(ctypes.byref(desc)))
-    nat.check(L.sinv_bcr_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()), "bcr")
+    if npart == 1:
+        nat.check(L.sinv_bcr_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()), "bcr")
+    else:
+        nat.check(L.sinv_bcr_parallel_batched(ctypes.byref(desc), int(npart), _p(ws), ws.numel(),
+                                              _stream()), "bcr_parallel")
     if check:
         nat.check_info(info)
     if return_fill:

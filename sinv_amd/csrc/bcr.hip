@@ -120,18 +120,35 @@ extern "C" long long sinv_bcr_fill_blocks(int nblocks) {
   return make_plan(nblocks, 1, 1).fill_blocks;
 }
 
-extern "C" int sinv_bcr_batched(const sinv_bcr_desc* d, void* work, size_t work_bytes,
-                                void* stream) {
-  if (!bcr_desc_ok(d)) {
-    set_error("sinv_bcr_batched: invalid descriptor");
-    return SINV_E_INVAL;
-  }
+namespace {
+
+// The BCR driver.  npart > 1 splits every level operation into one launch per
+// partition: partition p owns the padded block rows [p * stride, (p + 1) *
+// stride) (the last one the rest; bcr_utils.py:133-184 with stride =
+// (2^p - 1) / npart), and issues the operations anchored on its rows --
+// reducing its kept rows (their factors U / L, the A_jj update and the next
+// level's fill-in couplings; bcr_parallel.py:227-256), inverting its
+// eliminated rows, and producing its rows (bcr_parallel.py:1004-1015).  Each
+// operation family of a level is issued for every partition before the next
+// family, as the ranks exchange halo rows between them; the arithmetic and
+// its order per block are those of npart = 1, so results are bitwise equal.
+int bcr_run(const sinv_bcr_desc* d, int npart, void* work, size_t work_bytes, void* stream) {
   const int N = d->nblocks, b = d->blocksize, nb = d->batch;
   const Plan P = make_plan(N, b, nb);
   if (work == nullptr || work_bytes < P.bytes) {
     set_error("sinv_bcr_batched: workspace too small (%zu < %zu)", work_bytes, P.bytes);
     return SINV_E_INVAL;
   }
+  const long long npad = (1LL << (P.height + 1)) - 1;  // padded block rows
+  if (npart < 1 || npart > npad) {
+    set_error("sinv_bcr_parallel_batched: npart = %d must be in [1, %lld]", npart, npad);
+    return SINV_E_INVAL;
+  }
+  const long long pstride = npad / npart;
+  auto owner = [&](long long row) {
+    const long long o = row / pstride;
+    return (int)(o < npart ? o : npart - 1);
+  };
   hipStream_t s = (hipStream_t)stream;
   const long long b2 = (long long)b * b;
   cplx* base = (cplx*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
@@ -178,6 +195,30 @@ extern "C" int sinv_bcr_batched(const sinv_bcr_desc* d, void* work, size_t work_
     g.inner = count; g.outer = nb;
     return launch_gemm(g, Prec::F64, s);
   };
+  // Index range [0, count) of an operation family split into the runs owned by
+  // one partition (anchor(k) = block row of operation k, increasing in k);
+  // issue(k0, k1) launches one run.
+  auto split = [&](int count, auto anchor, auto issue) -> hipError_t {
+    int k0 = 0;
+    while (k0 < count) {
+      const int o = owner(anchor(k0));
+      int k1 = k0 + 1;
+      while (k1 < count && owner(anchor(k1)) == o) ++k1;
+      SINV_HIP_RET(issue(k0, k1));
+      k0 = k1;
+    }
+    return hipSuccess;
+  };
+  // gemm over index k of one family, run by run: operand offsets advance by
+  // their block steps
+  auto pgemm = [&](auto anchor, int count, Arr A, long long a0, long long as, Arr B, long long b0,
+                   long long bs, Arr C, long long c0, long long cs, cplx alpha, bool acc,
+                   Arr C0 = Arr{nullptr, 0}, long long c00 = 0, long long c0s = 0) {
+    return split(count, anchor, [&](int k0, int k1) {
+      return gemm(k1 - k0, A, a0 + k0 * as, as, B, b0 + k0 * bs, bs, C, c0 + k0 * cs, cs, alpha,
+                  acc, C0, c00 + k0 * c0s, c0s);
+    });
+  };
 
   if (d->info != nullptr) SINV_HIP_TRY(launch_fill_u64(d->info, 1, SINV_INFO_OK, s));
 
@@ -206,33 +247,43 @@ extern "C" int sinv_bcr_batched(const sinv_bcr_desc* d, void* work, size_t work_
     const cplx cs = l == 0 ? osc : one, mcs = l == 0 ? mosc : mone;
     const Arr Up = up_arr(l), Lo = lo_arr(l);
     const Arr U = warr(Lv.U), Lf = warr(Lv.L), Ai = warr(Lv.ainv);
-    // inv(A_tt) for eliminated rows r = 2q*s + s - 1  (bcr_serial.py:121,148)
-    SINV_HIP_TRY(launch_axpby_shift(b, b, one, Dw.p + (long long)(sl - 1) * b2, b, 2LL * sl * b2,
-                                    WI, W + Lv.ainv * b2, b, b2, WI, nullptr, nelim, nb, s));
-    // tag = row of the first eliminated block of the level; instances report z
-    SINV_HIP_TRY(launch_inverse(W + Lv.ainv * b2, b, b, b2, WI, nelim, nb, winv, d->info,
-                                sl - 1, s));
+    // block row of active index m at this level
+    auto rowm = [=](long long m) { return m * sl + sl - 1; };
+    auto elim = [=](int q) { return rowm(2LL * q); };      // eliminated row m = 2q
+    auto kept = [=](int q) { return rowm(2LL * q + 1); };  // kept row m = 2q+1
+    // inv(A_tt) for eliminated rows r = 2q*s + s - 1  (bcr_serial.py:121,148);
+    // tag = row of the first eliminated block of the run; instances report z
+    SINV_HIP_TRY(split(nelim, elim, [&](int q0, int q1) -> hipError_t {
+      SINV_HIP_RET(launch_axpby_shift(b, b, one, Dw.p + (sl - 1 + 2LL * sl * q0) * b2, b,
+                                      2LL * sl * b2, WI, W + (Lv.ainv + q0) * b2, b, b2, WI,
+                                      nullptr, q1 - q0, nb, s));
+      return launch_inverse(W + (Lv.ainv + q0) * b2, b, b, b2, WI, q1 - q0, nb, winv, d->info,
+                            (int)elim(q0), s);
+    }));
     const int ce = (n - 1 + 1) / 2;  // even couplings c = 2q < n-1
     const int co = (n - 1) / 2;      // odd couplings c = 2q+1 < n-1
     // c even: U_c = Ainv[q] Up[c], L_c = Lo[c] Ainv[q]      (bcr_serial.py:124-128)
-    SINV_HIP_TRY(gemm(ce, Ai, 0, 1, Up, 0, 2, U, 0, 2, cs, false));
-    SINV_HIP_TRY(gemm(ce, Lo, 0, 2, Ai, 0, 1, Lf, 0, 2, cs, false));
+    //   (computed by the partition of the kept row c + 1)
+    SINV_HIP_TRY(pgemm(kept, ce, Ai, 0, 1, Up, 0, 2, U, 0, 2, cs, false));
+    SINV_HIP_TRY(pgemm(kept, ce, Lo, 0, 2, Ai, 0, 1, Lf, 0, 2, cs, false));
     // c odd: U_c = Ainv[q+1] Lo[c], L_c = Up[c] Ainv[q+1]   (bcr_serial.py:150-154)
-    SINV_HIP_TRY(gemm(co, Ai, 1, 1, Lo, 1, 2, U, 1, 2, cs, false));
-    SINV_HIP_TRY(gemm(co, Up, 1, 2, Ai, 1, 1, Lf, 1, 2, cs, false));
+    //   (kept row c)
+    SINV_HIP_TRY(pgemm(kept, co, Ai, 1, 1, Lo, 1, 2, U, 1, 2, cs, false));
+    SINV_HIP_TRY(pgemm(kept, co, Up, 1, 2, Ai, 1, 1, Lf, 1, 2, cs, false));
     // kept rows m = 2q+1 (r = 2(q+1)s - 1): A_jj -= L_{m-1} Up[m-1]   (bcr_serial.py:130-133)
-    SINV_HIP_TRY(gemm(nkeep, Lf, 0, 2, Up, 0, 2, Arr{Dw.p + (long long)(2 * sl - 1) * b2, WI}, 0,
-                      2LL * sl, mcs, true));
+    SINV_HIP_TRY(pgemm(kept, nkeep, Lf, 0, 2, Up, 0, 2,
+                       Arr{Dw.p + (long long)(2 * sl - 1) * b2, WI}, 0, 2LL * sl, mcs, true));
     //   then A_jj -= L_m Lo[m] for m <= n-2                           (bcr_serial.py:156-159)
-    SINV_HIP_TRY(gemm(co, Lf, 1, 2, Lo, 1, 2, Arr{Dw.p + (long long)(2 * sl - 1) * b2, WI}, 0,
-                      2LL * sl, mcs, true));
+    SINV_HIP_TRY(pgemm(kept, co, Lf, 1, 2, Lo, 1, 2, Arr{Dw.p + (long long)(2 * sl - 1) * b2, WI},
+                       0, 2LL * sl, mcs, true));
     // next level's couplings (fill-in, bcr_serial.py:137-143, 163-169)
     const Level& Nx = P.lv[l + 1];
     const int ncn = Nx.n > 1 ? Nx.n - 1 : 0;
-    //   Up_{l+1}[c'] = -L_{2c'+1} Up[2c'+2]
-    SINV_HIP_TRY(gemm(ncn, Lf, 1, 2, Up, 2, 2, warr(Nx.up), 0, 1, mcs, false));
-    //   Lo_{l+1}[c'] = -L_{2c'+2} Lo[2c'+1]
-    SINV_HIP_TRY(gemm(ncn, Lf, 2, 2, Lo, 1, 2, warr(Nx.lo), 0, 1, mcs, false));
+    //   Up_{l+1}[c'] = -L_{2c'+1} Up[2c'+2]      (kept row 2c'+1)
+    SINV_HIP_TRY(pgemm(kept, ncn, Lf, 1, 2, Up, 2, 2, warr(Nx.up), 0, 1, mcs, false));
+    //   Lo_{l+1}[c'] = -L_{2c'+2} Lo[2c'+1]      (kept row 2c'+3)
+    SINV_HIP_TRY(pgemm([=](int q) { return kept(q + 1); }, ncn, Lf, 2, 2, Lo, 1, 2, warr(Nx.lo),
+                       0, 1, mcs, false));
   }
 
   // ---------------------------------------------------------------- root (bcr_serial.py:248-249)
@@ -265,23 +316,26 @@ extern "C" int sinv_bcr_batched(const sinv_bcr_desc* d, void* work, size_t work_
       SINV_HIP_TRY(gemm(1, Gup, 0, 0, Lf, 0, 0, GD, gd(0), 0, mone, true, Ai, 0, 0));
     }
     // -- centers m = 2q, q = 1..nc (a = m-1, c = m+1)        (bcr_serial.py:354-383)
+    //   (produced by the partition of row t = r_m, m = 2k + 2 for run index k)
     const int ncen = (n - 2) / 2;  // even m in [2, n-2]
+    auto tcen = [=](int k) { return gd(2LL * k + 2); };
     if (ncen > 0) {
       // G[a,t] = Gup[m-1] = -G_aa L_{m-1} - G_ac L_m      with G_ac = Gup1[q-1]
-      SINV_HIP_TRY(gemm(ncen, GD, gd(1), 2LL * sl, Lf, 1, 2, Gup, 1, 2, mone, false));
-      SINV_HIP_TRY(gemm(ncen, Gup1, 0, 1, Lf, 2, 2, Gup, 1, 2, mone, true));
+      SINV_HIP_TRY(pgemm(tcen, ncen, GD, gd(1), 2LL * sl, Lf, 1, 2, Gup, 1, 2, mone, false));
+      SINV_HIP_TRY(pgemm(tcen, ncen, Gup1, 0, 1, Lf, 2, 2, Gup, 1, 2, mone, true));
       // G[c,t] = Glo[m] = -G_ca L_{m-1} - G_cc L_m        with G_ca = Glo1[q-1]
-      SINV_HIP_TRY(gemm(ncen, Glo1, 0, 1, Lf, 1, 2, Glo, 2, 2, mone, false));
-      SINV_HIP_TRY(gemm(ncen, GD, gd(3), 2LL * sl, Lf, 2, 2, Glo, 2, 2, mone, true));
+      SINV_HIP_TRY(pgemm(tcen, ncen, Glo1, 0, 1, Lf, 1, 2, Glo, 2, 2, mone, false));
+      SINV_HIP_TRY(pgemm(tcen, ncen, GD, gd(3), 2LL * sl, Lf, 2, 2, Glo, 2, 2, mone, true));
       // G[t,a] = Glo[m-1] = -U_{m-1} G_aa - U_m G_ca
-      SINV_HIP_TRY(gemm(ncen, U, 1, 2, GD, gd(1), 2LL * sl, Glo, 1, 2, mone, false));
-      SINV_HIP_TRY(gemm(ncen, U, 2, 2, Glo1, 0, 1, Glo, 1, 2, mone, true));
+      SINV_HIP_TRY(pgemm(tcen, ncen, U, 1, 2, GD, gd(1), 2LL * sl, Glo, 1, 2, mone, false));
+      SINV_HIP_TRY(pgemm(tcen, ncen, U, 2, 2, Glo1, 0, 1, Glo, 1, 2, mone, true));
       // G[t,c] = Gup[m] = -U_{m-1} G_ac - U_m G_cc
-      SINV_HIP_TRY(gemm(ncen, U, 1, 2, Gup1, 0, 1, Gup, 2, 2, mone, false));
-      SINV_HIP_TRY(gemm(ncen, U, 2, 2, GD, gd(3), 2LL * sl, Gup, 2, 2, mone, true));
+      SINV_HIP_TRY(pgemm(tcen, ncen, U, 1, 2, Gup1, 0, 1, Gup, 2, 2, mone, false));
+      SINV_HIP_TRY(pgemm(tcen, ncen, U, 2, 2, GD, gd(3), 2LL * sl, Gup, 2, 2, mone, true));
       // G[t,t] = Ainv[q] - Glo[m-1] L_{m-1} - Gup[m] L_m
-      SINV_HIP_TRY(gemm(ncen, Glo, 1, 2, Lf, 1, 2, GD, gd(2), 2LL * sl, mone, true, Ai, 1, 1));
-      SINV_HIP_TRY(gemm(ncen, Gup, 2, 2, Lf, 2, 2, GD, gd(2), 2LL * sl, mone, true));
+      SINV_HIP_TRY(pgemm(tcen, ncen, Glo, 1, 2, Lf, 1, 2, GD, gd(2), 2LL * sl, mone, true, Ai, 1,
+                         1));
+      SINV_HIP_TRY(pgemm(tcen, ncen, Gup, 2, 2, Lf, 2, 2, GD, gd(2), 2LL * sl, mone, true));
     }
     // -- corner from above: last eliminated row m = n-1 when n is odd and n >= 3
     if (n % 2 == 1 && n >= 3) {
@@ -311,4 +365,24 @@ extern "C" int sinv_bcr_batched(const sinv_bcr_desc* d, void* work, size_t work_
     }
   }
   return SINV_OK;
+}
+
+}  // namespace
+
+extern "C" int sinv_bcr_batched(const sinv_bcr_desc* d, void* work, size_t work_bytes,
+                                void* stream) {
+  if (!bcr_desc_ok(d)) {
+    set_error("sinv_bcr_batched: invalid descriptor");
+    return SINV_E_INVAL;
+  }
+  return bcr_run(d, 1, work, work_bytes, stream);
+}
+
+extern "C" int sinv_bcr_parallel_batched(const sinv_bcr_desc* d, int npart, void* work,
+                                         size_t work_bytes, void* stream) {
+  if (!bcr_desc_ok(d)) {
+    set_error("sinv_bcr_parallel_batched: invalid descriptor");
+    return SINV_E_INVAL;
+  }
+  return bcr_run(d, npart, work, work_bytes, stream);
 }

@@ -127,3 +127,35 @@ def test_bcr_parallel_dropin_reference_shapes(built_lib, n, b, cplx, sym, nparts
     for x, r in zip(matu.convertDenseToBlkTridiag(G, b), matu.convertDenseToBlkTridiag(R, b)):
         assert np.allclose(x, r)
     assert np.array_equal(G, bcr_s.bcr_serial(A, b))
+
+
+@pytest.mark.parametrize("N,P", [(31, 2), (31, 4), (20, 3), (45, 8)])
+def test_bcr_parallel_launches_follow_the_rank_split(built_lib, N, P):
+    """With P partitions every level's eliminated-row inverses run as one
+    launch per partition that owns eliminated rows of that level (the ranks'
+    rows of bcr_utils.divide_matrix on the padded rows), and the produced
+    blocks equal BCR-serial's bit for bit."""
+    import torch
+    from sinv_amd import _native as nat, core
+    from sinv_amd.algorithms import bcr_p
+    from sinv_amd.utils import configs
+    d, u, l = configs.dominant_blocks(N, 16, seed=N + P)
+    ref = core.bcr_batched(d, u, l, return_fill=True)
+    torch.cuda.synchronize()
+    with nat.LaunchProfiler() as prof:
+        got = core.bcr_batched(d, u, l, return_fill=True, npart=P)
+        torch.cuda.synchronize()
+    for x, y in zip(got, ref):
+        assert torch.equal(x, y)
+    inv_launches = sum(1 for kind, _, _ in prof.records() if kind == "inv_leaf")
+    npad = 2 ** int(np.ceil(np.log2(N + 1))) - 1
+    starts, sizes = bcr_p.divide_matrix(npad, P)
+    stride = sizes[0]
+    expect = 1  # the root
+    s = 1
+    while 2 * s - 1 < npad:
+        elim = [m * s + s - 1 for m in range(0, N // s, 2)]   # eliminated real rows of the level
+        if N // s > 1:
+            expect += len({min(r // stride, P - 1) for r in elim})
+        s *= 2
+    assert inv_launches == expect, (inv_launches, expect)
