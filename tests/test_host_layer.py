@@ -21,6 +21,66 @@ def test_generators_match_reference_streams():
             assert np.array_equal(a, b)
 
 
+def _golden_generators():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "generator_vectors.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def _unpack(x):
+    a = np.asarray(x["re"])
+    return a + 1j * np.asarray(x["im"]) if "im" in x else a
+
+
+def test_generators_match_the_references_frozen_streams():
+    """Frozen outputs of the reference's own matrices_utils.py on its tests'
+    shapes (tests/golden/make_generator_vectors.py): every generated matrix of
+    sinv_amd.utils.matu and of the oracle has the same bytes (SHA-256), dtype
+    and shape, for the banded generator (with the tests' float bandwidth
+    ceil(b / 2) as well) and the dense one."""
+    import hashlib
+    g = _golden_generators()
+    assert len(g["banded"]) == 72 and len(g["random"]) == 16
+    for c in g["banded"]:
+        for impl in (matu, mu):
+            for bw in (c["bandwidth"], float(c["bandwidth"])):
+                A = impl.generateBandedDiagonalMatrix(c["size"], bw, c["is_complex"],
+                                                      c["is_symmetric"], c["seed"])
+                assert str(A.dtype) == c["dtype"] and list(A.shape) == c["shape"]
+                assert hashlib.sha256(np.ascontiguousarray(A).tobytes()).hexdigest() == c["sha256"], c
+    for c in g["random"]:
+        for impl in (matu, mu):
+            A = impl.generateRandomNumpyMat(c["size"], c["is_complex"], c["is_symmetric"], c["seed"])
+            assert hashlib.sha256(np.ascontiguousarray(A).tobytes()).hexdigest() == c["sha256"], c
+            assert np.array_equal(A[0], _unpack(c["first_row"]))
+
+
+def test_converters_match_the_references_frozen_outputs():
+    g = _golden_generators()
+    for c in g["convert"]:
+        A = _unpack(c["A"])
+        d, u, l = matu.convertDenseToBlkTridiag(A, c["blocksize"])
+        for got, key in ((d, "diag"), (u, "upper"), (l, "lower")):
+            assert np.array_equal(np.asarray(got), _unpack(c[key])), key
+        assert np.array_equal(matu.convertBlkTridiagToDense(d, u, l), _unpack(c["dense"]))
+
+
+def test_golden_pin_catches_a_shifted_stream():
+    """The pin is not vacuous: drawing the imaginary part before the real one
+    (a plausible slip in the complex generator) changes the bytes."""
+    import hashlib
+    c = next(x for x in _golden_generators()["banded"] if x["is_complex"] and x["size"] == 36)
+    np.random.seed(c["seed"])
+    n = c["size"]
+    im = np.random.rand(n, n)
+    re = np.random.rand(n, n)
+    A = re + 1j * im
+    assert hashlib.sha256(A.tobytes()).hexdigest() != c["sha256"]
+
+
 def test_banded_mask_against_loop():
     """matrices_utils.py:86-89 loops over all entries; the vectorised mask agrees."""
     A = matu.generateBandedDiagonalMatrix(23, 4.0, True, False, 5)
