@@ -306,6 +306,19 @@ int sinv_zinv_batched(void* A, int n, int lda, long long strideA, int batch, voi
 #define SINV_LEAF_PANEL16_FLAGS 5
 #define SINV_LEAF_PANEL8_FLAGS 6
 int sinv_leaf_force_variant(int variant);
+/* Tall-panel kernel of the blocked inverse for blocks above 64 (no
+ * counterpart in the reference; for tests and measurements).  SINV_GJ_STEP
+ * (default): one barrier per eliminated column, every wave applying each step
+ * through one code path, panels of 64 columns up to 384 rows, 32 up to 512,
+ * 16 up to 896, 8 above.  SINV_GJ_PANEL: one barrier per wave's group of
+ * columns, the owner factoring them alone, panels of 64 columns up to 128
+ * rows, 32 up to 256, 16 up to 512, 8 above.  Both pivot identically (the
+ * exact first maximum of |re| + |im| in position order) and agree to
+ * rounding.  Process-wide, unsynchronised; SINV_GJ_PANEL=panel in the
+ * environment sets the initial choice. */
+#define SINV_GJ_STEP 0
+#define SINV_GJ_PANEL 1
+int sinv_gj_force_variant(int variant);
 /* Bound on a leaf wave's wait for a panel hand-off (no counterpart in the
  * reference; for failure-handling tests).  polls = -1 restores the default
  * (2^22); a wave that exhausts it stops and its block reports as singular

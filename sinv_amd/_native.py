@@ -171,6 +171,8 @@ def lib():
     L.sinv_leaf_force_variant.argtypes = [_i]
     L.sinv_leaf_spin_limit.restype = _i
     L.sinv_leaf_spin_limit.argtypes = [_i]
+    L.sinv_gj_force_variant.restype = _i
+    L.sinv_gj_force_variant.argtypes = [_i]
     L.sinv_gemm_tune_set.restype = _i
     L.sinv_gemm_tune_set.argtypes = [_i, _i, _i, _ll, _i, _i, _i, _i, _i]
     L.sinv_gemm_tune_clear.restype = _i

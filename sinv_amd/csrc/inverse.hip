@@ -927,6 +927,125 @@ __global__ __launch_bounds__(64 * kGjWaves) void gj_panel_kernel(
     atomicMin(info, ((unsigned long long)(unsigned)tag << 32) | (unsigned long long)(unsigned)z);
 }
 
+// Step form of the tall-panel kernel: the same layout (8 waves, lane l holds
+// rows l + 64 s, wave v columns v R .. v R + R - 1) with one barrier per
+// column instead of per panel.  The owner of column k finds the pivot and
+// publishes the multipliers g and the pivot row index; after the barrier every
+// wave, the owner included, applies step k to its columns with one code path
+// (the pivot column itself is taken as cleared with a unit pivot-row entry,
+// which writes the inverse's column -g there).  The critical path per column
+// is search, publish, barrier and the next owner's update; the wave that owns
+// the next column applies at raised priority.
+template <int M, int R>
+__global__ __launch_bounds__(64 * kGjWaves) void gj_step_kernel(
+    cplx* A, int n, int lda, long long s_in, long long s_out, int inner, int c0, int wc,
+    int* ipos, int* ipiv, unsigned long long* info, int tag) {
+  __shared__ cplx gbuf[2][M * 64];  // multipliers of the step, two steps
+  __shared__ int pinfo[2][2];        // pivot row and its position before the step
+  __shared__ int pivl[kGjWaves * R];
+  const int z = blockIdx.x;
+  const int zo = z / inner, zi = z - zo * inner;
+  A += zi * s_in + zo * s_out;
+  ipos += (long long)z * n;
+  ipiv += (long long)z * n;
+  const int lane = threadIdx.x & 63;
+  const int v = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
+  cplx a[M][R];
+  int pos[M];
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
+    const int row = lane + 64 * s;
+    const cplx* src = A + (long long)row * lda + c0 + v * R;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (row < n && v * R + r < wc) {
+        const double2 t = *reinterpret_cast<const double2*>(src + r);
+        a[s][r] = cplx{t.x, t.y};
+      } else {
+        a[s][r] = cplx{0.0, 0.0};
+      }
+    }
+    pos[s] = row < n ? (c0 == 0 ? row : ipos[row]) : -1;
+  }
+
+  bool singular = false;
+  for (int kb = 0; kb * R < wc; ++kb) {
+#pragma unroll
+    for (int kr = 0; kr < R; ++kr) {
+      const int k = kb * R + kr;
+      if (k >= wc) break;  // uniform
+      const int buf = k & 1;
+      const int kg = c0 + k;
+      if (v == kb) {
+        __builtin_amdgcn_s_setprio(3);
+        int lp, sp, pp;
+        bool zero;
+        gj_pivot<M, R>(a, pos, kr, kg, n, lane, lp, sp, pp, zero);
+        singular |= zero;
+        const int p = lp + 64 * sp;
+        cplx piv{0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < M; ++s)
+          if (s == sp) piv = cplx{readlane_f64(a[s][kr].re, lp), readlane_f64(a[s][kr].im, lp)};
+        const cplx d = crecip_uniform(piv);
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+          const cplx gf = cmul(a[s][kr], d);
+          const bool is_p = lane + 64 * s == p;
+          gbuf[buf][s * 64 + lane] = cplx{is_p ? -d.re : gf.re, is_p ? -d.im : gf.im};
+        }
+        if (lane == 0) {
+          pinfo[buf][0] = p;
+          pinfo[buf][1] = pp;
+          pivl[k] = p;
+        }
+      }
+      __syncthreads();
+      // every wave: step k on its columns, the next column's owner at raised priority
+      if (v == (k + 1) / R)
+        __builtin_amdgcn_s_setprio(2);
+      else
+        __builtin_amdgcn_s_setprio(0);
+      const int p = __builtin_amdgcn_readfirstlane(pinfo[buf][0]);
+      const int pp = __builtin_amdgcn_readfirstlane(pinfo[buf][1]);
+      cplx P[R];
+      gj_row<M, R>(a, p >> 6, p & 63, P);
+      const bool mine = v == kb;
+#pragma unroll
+      for (int s = 0; s < M; ++s) {
+        const cplx g = gbuf[buf][s * 64 + lane];
+        const bool is_p = lane + 64 * s == p;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          // the pivot column: cleared, with a unit pivot-row entry (writes -g)
+          const bool pc = r == kr && mine;
+          a[s][r] = gj_upd(a[s][r], g, pc ? cplx{1.0, 0.0} : P[r], is_p || pc);
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < M; ++s)
+        pos[s] = pos[s] == kg ? pp : (lane + 64 * s == p ? kg : pos[s]);
+    }
+  }
+
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
+    const int row = lane + 64 * s;
+    if (row < n) {
+      cplx* dst = A + (long long)row * lda + c0 + v * R;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (v * R + r < wc) *reinterpret_cast<double2*>(dst + r) = double2{a[s][r].re, a[s][r].im};
+      if (v == 0) ipos[row] = pos[s];
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < wc) ipiv[c0 + threadIdx.x] = pivl[threadIdx.x];
+  if (lane == 0 && singular && info != nullptr)
+    atomicMin(info, ((unsigned long long)(unsigned)tag << 32) | (unsigned long long)(unsigned)z);
+}
+
 // Panels of blocks above 64 M = 2048 rows: the panel (16 columns) stays in
 // global memory, where it is L2-resident, and is walked 64 rows at a time per
 // step (16 lanes per row, lane j on column j; one workgroup per matrix).  The
@@ -1161,11 +1280,27 @@ __global__ __launch_bounds__(256) void gj_permute_kernel(const cplx* S, int lds,
 struct GjShape {
   int M, R;
 };
+// Panel kernel choice: the per-column kernel (default) or the per-panel one
+// (SINV_GJ_PANEL=panel, or sinv_gj_force_variant); each has its own widest
+// panels that fit its registers without spilling.
+int g_gj_variant = -1;  // SINV_GJ_STEP / SINV_GJ_PANEL; -1 until first read
+bool gj_use_step() {
+  if (g_gj_variant < 0) {
+    const char* e = getenv("SINV_GJ_PANEL");
+    g_gj_variant = (e != nullptr && strcmp(e, "panel") == 0) ? SINV_GJ_PANEL : SINV_GJ_STEP;
+  }
+  return g_gj_variant == SINV_GJ_STEP;
+}
+
 GjShape gj_shape(int n) {
   static constexpr int kMs[] = {2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32};
   const int m = (n + 63) / 64;
+  const bool step = gj_use_step();
   for (int M : kMs)
-    if (M >= m) return GjShape{M, M <= 2 ? 8 : (M <= 4 ? 4 : (M <= 8 ? 2 : 1))};
+    if (M >= m) {
+      if (step) return GjShape{M, M <= 6 ? 8 : (M <= 8 ? 4 : (M <= 14 ? 2 : 1))};
+      return GjShape{M, M <= 2 ? 8 : (M <= 4 ? 4 : (M <= 8 ? 2 : 1))};
+    }
   return GjShape{0, 0};
 }
 int gj_width(int n) {
@@ -1173,12 +1308,16 @@ int gj_width(int n) {
   return g.M > 0 ? kGjWaves * g.R : kGjGlobalW;
 }
 
-template <int M, int R>
+template <int M, int R, bool STEP>
 void gj_panel_mr(cplx* A, int n, int lda, long long s_in, long long s_out, int inner, unsigned nb,
                  int c0, int wc, int* ipos, int* ipiv, unsigned long long* info, int tag,
                  hipStream_t s) {
-  hipLaunchKernelGGL((gj_panel_kernel<M, R>), dim3(nb), dim3(64 * kGjWaves), 0, s, A, n, lda,
-                     s_in, s_out, inner, c0, wc, ipos, ipiv, info, tag);
+  if constexpr (STEP)
+    hipLaunchKernelGGL((gj_step_kernel<M, R>), dim3(nb), dim3(64 * kGjWaves), 0, s, A, n, lda,
+                       s_in, s_out, inner, c0, wc, ipos, ipiv, info, tag);
+  else
+    hipLaunchKernelGGL((gj_panel_kernel<M, R>), dim3(nb), dim3(64 * kGjWaves), 0, s, A, n, lda,
+                       s_in, s_out, inner, c0, wc, ipos, ipiv, info, tag);
 }
 
 hipError_t launch_gj_panel(cplx* A, int n, int lda, long long s_in, long long s_out, int inner,
@@ -1186,30 +1325,55 @@ hipError_t launch_gj_panel(cplx* A, int n, int lda, long long s_in, long long s_
                            unsigned long long* info, int tag, hipStream_t s) {
   const unsigned nb = (unsigned)(inner * outer);
   const int pid = prof_open(s);
-#define SINV_GJ_CASE(M_, R_)                                                           \
-  case M_:                                                                             \
-    gj_panel_mr<M_, R_>(A, n, lda, s_in, s_out, inner, nb, c0, wc, ipos, ipiv, info, tag, \
-                        s);                                                            \
+#define SINV_GJ_CASE(M_, R_, STEP_)                                                     \
+  case M_ * 100 + R_:                                                                  \
+    gj_panel_mr<M_, R_, STEP_>(A, n, lda, s_in, s_out, inner, nb, c0, wc, ipos, ipiv,   \
+                               info, tag, s);                                          \
     break;
-  switch (gj_shape(n).M) {
-    SINV_GJ_CASE(2, 8)
-    SINV_GJ_CASE(3, 4)
-    SINV_GJ_CASE(4, 4)
-    SINV_GJ_CASE(5, 2)
-    SINV_GJ_CASE(6, 2)
-    SINV_GJ_CASE(7, 2)
-    SINV_GJ_CASE(8, 2)
-    SINV_GJ_CASE(10, 1)
-    SINV_GJ_CASE(12, 1)
-    SINV_GJ_CASE(14, 1)
-    SINV_GJ_CASE(16, 1)
-    SINV_GJ_CASE(20, 1)
-    SINV_GJ_CASE(24, 1)
-    SINV_GJ_CASE(28, 1)
-    SINV_GJ_CASE(32, 1)
-    default:
-      hipLaunchKernelGGL(gj_panel_global_kernel, dim3(nb), dim3(1024), 2 * n * sizeof(int), s, A,
-                         n, lda, s_in, s_out, inner, c0, wc, ipos, ipiv, info, tag);
+  const GjShape gs = gj_shape(n);
+  if (gs.M == 0) {
+    hipLaunchKernelGGL(gj_panel_global_kernel, dim3(nb), dim3(1024), 2 * n * sizeof(int), s, A,
+                       n, lda, s_in, s_out, inner, c0, wc, ipos, ipiv, info, tag);
+  } else if (gj_use_step()) {
+    switch (gs.M * 100 + gs.R) {
+      SINV_GJ_CASE(2, 8, true)
+      SINV_GJ_CASE(3, 8, true)
+      SINV_GJ_CASE(4, 8, true)
+      SINV_GJ_CASE(5, 8, true)
+      SINV_GJ_CASE(6, 8, true)
+      SINV_GJ_CASE(7, 4, true)
+      SINV_GJ_CASE(8, 4, true)
+      SINV_GJ_CASE(10, 2, true)
+      SINV_GJ_CASE(12, 2, true)
+      SINV_GJ_CASE(14, 2, true)
+      SINV_GJ_CASE(16, 1, true)
+      SINV_GJ_CASE(20, 1, true)
+      SINV_GJ_CASE(24, 1, true)
+      SINV_GJ_CASE(28, 1, true)
+      SINV_GJ_CASE(32, 1, true)
+      default:
+        return hipErrorInvalidValue;
+    }
+  } else {
+    switch (gs.M * 100 + gs.R) {
+      SINV_GJ_CASE(2, 8, false)
+      SINV_GJ_CASE(3, 4, false)
+      SINV_GJ_CASE(4, 4, false)
+      SINV_GJ_CASE(5, 2, false)
+      SINV_GJ_CASE(6, 2, false)
+      SINV_GJ_CASE(7, 2, false)
+      SINV_GJ_CASE(8, 2, false)
+      SINV_GJ_CASE(10, 1, false)
+      SINV_GJ_CASE(12, 1, false)
+      SINV_GJ_CASE(14, 1, false)
+      SINV_GJ_CASE(16, 1, false)
+      SINV_GJ_CASE(20, 1, false)
+      SINV_GJ_CASE(24, 1, false)
+      SINV_GJ_CASE(28, 1, false)
+      SINV_GJ_CASE(32, 1, false)
+      default:
+        return hipErrorInvalidValue;
+    }
   }
 #undef SINV_GJ_CASE
   prof_close(pid, s, PROF_INV_PANEL, 8.0 * n * (double)wc * wc * nb,
@@ -1403,6 +1567,16 @@ extern "C" int sinv_leaf_force_variant(int variant) {
     return SINV_E_INVAL;
   }
   g_leaf_variant = variant;
+  return SINV_OK;
+}
+
+extern "C" int sinv_gj_force_variant(int variant) {
+  using namespace sinv;
+  if (variant != SINV_GJ_STEP && variant != SINV_GJ_PANEL) {
+    set_error("sinv_gj_force_variant: unknown variant %d", variant);
+    return SINV_E_INVAL;
+  }
+  g_gj_variant = variant;
   return SINV_OK;
 }
 

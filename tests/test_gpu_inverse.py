@@ -128,3 +128,27 @@ def test_zinv_singular_reported(built_lib, n):
     with pytest.raises(np.linalg.LinAlgError) as ei:
         core.zinv_batched(A)
     assert ei.value.instance == 1
+
+
+@pytest.mark.parametrize("n", [100, 256, 384, 416, 512, 1100])
+def test_zinv_both_panel_kernels(built_lib, n):
+    """The per-column and per-panel tall-panel kernels (different panel widths,
+    so different sweep shapes) both give numpy's inverse, pivoting over all
+    rows, and agree with each other to rounding."""
+    from sinv_amd import _native as nat, core
+    L = nat.lib()
+    rng = np.random.default_rng(77 + n)
+    A = _rand(rng, 2, n, n)
+    A[1, 0, :] *= 1e-3
+    out = {}
+    try:
+        for v in (1, 0):   # SINV_GJ_PANEL, SINV_GJ_STEP
+            assert L.sinv_gj_force_variant(v) == 0
+            out[v] = core.zinv_batched(A).cpu().numpy()
+    finally:
+        L.sinv_gj_force_variant(0)
+    for z in range(2):
+        for v in (0, 1):
+            _check(out[v][z], A[z], (n, z, v))
+        assert np.linalg.norm(out[0][z] - out[1][z]) / np.linalg.norm(out[1][z]) < \
+            1e-15 * np.linalg.cond(A[z]) * n
