@@ -709,11 +709,12 @@ __device__ __forceinline__ void gj_row(const cplx (&a)[M][R], int slot, int plan
     }
 }
 
-// a - g P in four FMAs, with a taken as zero on the pivot row (whose entries
-// are replaced: g = -1 / pivot there, so the row becomes P / pivot)
-__device__ __forceinline__ cplx gj_upd(cplx a, cplx g, cplx P, bool is_p) {
-  const double are = is_p ? 0.0 : a.re, aim = is_p ? 0.0 : a.im;
-  return cplx{fma(-g.re, P.re, fma(g.im, P.im, are)), fma(-g.re, P.im, fma(-g.im, P.re, aim))};
+// keep a - g P, keep = 0 on the pivot row (whose entries are replaced:
+// g = -1 / pivot there, so the row becomes P / pivot) and 1 elsewhere: six
+// FP64 operations, the 0/1 factor folded into an FMA instead of selects
+__device__ __forceinline__ cplx gj_upd(cplx a, cplx g, cplx P, double keep) {
+  const double ti = g.im * P.im, tr = g.im * P.re;
+  return cplx{fma(-g.re, P.re, fma(a.re, keep, ti)), fma(-g.re, P.im, fma(a.im, keep, -tr))};
 }
 
 // Pivot of global column kg, held in this wave's column kr: the first maximum
@@ -805,7 +806,8 @@ __device__ __forceinline__ void gj_factor(cplx (&a)[M][R], int (&pos)[M], int kb
       for (int j = 1; j < R; ++j) {
         const int r = (kr + j) % R;  // the next pivot column first
 #pragma unroll
-        for (int s = 0; s < M; ++s) a[s][r] = gj_upd(a[s][r], g[s], P[r], lane + 64 * s == p);
+        for (int s = 0; s < M; ++s)
+          a[s][r] = gj_upd(a[s][r], g[s], P[r], lane + 64 * s == p ? 0.0 : 1.0);
       }
 #pragma unroll
       for (int s = 0; s < M; ++s) pos[s] = pos[s] == kg ? pp : (lane + 64 * s == p ? kg : pos[s]);
@@ -837,7 +839,7 @@ __device__ __forceinline__ void gj_apply(cplx (&a)[M][R], int (&pos)[M], int kb,
         const cplx g = gb[(kr * M + s) * 64 + lane];
         const bool is_p = lane + 64 * s == p;
 #pragma unroll
-        for (int r = 0; r < R; ++r) a[s][r] = gj_upd(a[s][r], g, P[r], is_p);
+        for (int r = 0; r < R; ++r) a[s][r] = gj_upd(a[s][r], g, P[r], is_p ? 0.0 : 1.0);
       }
 #pragma unroll
       for (int s = 0; s < M; ++s) pos[s] = pos[s] == kg ? pp : (lane + 64 * s == p ? kg : pos[s]);
@@ -1015,12 +1017,12 @@ __global__ __launch_bounds__(64 * kGjWaves) void gj_step_kernel(
 #pragma unroll
       for (int s = 0; s < M; ++s) {
         const cplx g = gbuf[buf][s * 64 + lane];
-        const bool is_p = lane + 64 * s == p;
+        const double keep = lane + 64 * s == p ? 0.0 : 1.0;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           // the pivot column: cleared, with a unit pivot-row entry (writes -g)
           const bool pc = r == kr && mine;
-          a[s][r] = gj_upd(a[s][r], g, pc ? cplx{1.0, 0.0} : P[r], is_p || pc);
+          a[s][r] = gj_upd(a[s][r], g, pc ? cplx{1.0, 0.0} : P[r], pc ? 0.0 : keep);
         }
       }
 #pragma unroll
