@@ -307,19 +307,22 @@ int sinv_zinv_batched(void* A, int n, int lda, long long strideA, int batch, voi
 #define SINV_LEAF_PANEL8_FLAGS 6
 int sinv_leaf_force_variant(int variant);
 /* Tall-panel kernel of the blocked inverse for blocks above 64 (no
- * counterpart in the reference; for tests and measurements).  SINV_GJ_STEP
- * (default): one barrier per eliminated column, every wave applying each step
- * through one code path, panels of 64 columns up to 384 rows, 32 up to 512,
- * 16 up to 896, 8 above.  SINV_GJ_PANEL: one barrier per wave's group of
- * columns, the owner factoring them alone, panels of 64 columns up to 128
- * rows, 32 up to 256, 16 up to 512, 8 above.  Both pivot identically (the
+ * counterpart in the reference; for tests and measurements).  SINV_GJ_FLOW
+ * (default): steps handed between waves through LDS flags, the next pivot
+ * column updated and searched first, pivot rows scaled at write-back (the
+ * step kernel serves blocks above 1024).  SINV_GJ_STEP: one barrier per
+ * eliminated column.  Both use panels of 64 columns up to 384 rows, 32 up to
+ * 512, 16 up to 896, 8 above.  SINV_GJ_PANEL: one barrier per wave's group
+ * of columns, the owner factoring them alone, panels of 64 columns up to 128
+ * rows, 32 up to 256, 16 up to 512, 8 above.  All pivot identically (the
  * exact first maximum of |re| + |im| in position order) and agree to
- * rounding.  Process-wide, unsynchronised; SINV_GJ_PANEL=panel in the
- * environment sets the initial choice. */
+ * rounding.  Process-wide, unsynchronised; SINV_GJ_PANEL=flow / step / panel
+ * in the environment sets the initial choice. */
 #define SINV_GJ_STEP 0
 #define SINV_GJ_PANEL 1
+#define SINV_GJ_FLOW 2
 int sinv_gj_force_variant(int variant);
-/* Bound on a leaf wave's wait for a panel hand-off (no counterpart in the
+/* Bound on a leaf or flow tall-panel wave's wait for a hand-off (no counterpart in the
  * reference; for failure-handling tests).  polls = -1 restores the default
  * (2^22); a wave that exhausts it stops and its block reports as singular
  * through info, so the kernel always exits. */

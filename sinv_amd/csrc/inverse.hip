@@ -1048,6 +1048,315 @@ __global__ __launch_bounds__(64 * kGjWaves) void gj_step_kernel(
     atomicMin(info, ((unsigned long long)(unsigned)tag << 32) | (unsigned long long)(unsigned)z);
 }
 
+// Flow form of the tall-panel kernel (the default up to 1024 rows): 8 waves,
+// lane l holds rows l + 64 s (s < M); the panel's columns are dealt round
+// robin, wave v holding columns v + 8 j (j < R), so consecutive steps are
+// produced by different waves.  Steps are handed from wave to wave through
+// LDS slots and flags instead of barriers:
+//  * Step k (column k, owned by wave k % 8) is published in slot k % NS: the
+//    multipliers g (0 on the pivot row), the pivot row's index and a sequence
+//    flag (k + 1, release); every wave applies it (acquire) and then counts
+//    itself out of the slot, and the producer of step k + NS waits for all
+//    eight before reusing it.
+//  * The wave that produces step k + 1 applies step k to that column first,
+//    then searches and publishes step k + 1, then applies step k to its
+//    other columns; the other waves' updates are off that path.
+//  * The update is a <- a - g P on every row: the pivot row keeps its values
+//    unscaled (g = 0 there) and is scaled by 1 / pivot when the panel is
+//    written back; linear updates commute with that scaling.  Four FP64 FMAs
+//    per element.  The pivot column becomes -g (1 on the pivot row), scaled
+//    the same way.
+//  * Every wave tracks the rows already used as pivots in one 64-bit scalar
+//    mask per slot, so the pivot search needs no positions: the largest
+//    modulus is one max over 32-bit keys (the high word of |re| + |im|); only
+//    exact ties read positions, to take the first in position order, as
+//    izamax does.  Row positions live once in LDS, moved by the producer.
+// Every wait is bounded (spin_limit polls); a wave that runs out stops and
+// the matrix is reported through info like a singular one.
+template <int M>
+struct GjFlow {
+  static constexpr int NS = M <= 8 ? 4 : 3;  // slots: at most 48 KB of multipliers
+};
+constexpr int kGjFlowMaxM = 16;
+
+__device__ __forceinline__ bool gjf_wait_geq(const int* flag, int target, int spin_limit) {
+  int polls = 0;
+  while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < target) {
+    if (++polls > spin_limit) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  return true;
+}
+
+// Step k (global column kg = c0 + k) from this wave's column j.
+template <int M, int R, int NS>
+__device__ __forceinline__ void gjf_produce(cplx (&a)[M][R], int j, int k, int c0, int n,
+                                            int lane, const unsigned long long (&used)[M],
+                                            cplx* gs, int* pinfo, int* seq, int* cnt, int* spos,
+                                            int* rowat, cplx* dv, int spin_limit,
+                                            bool& singular, bool& stuck) {
+  const int kg = c0 + k;
+  const int slot = k % NS;
+  const int freed = __hip_atomic_load(&cnt[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  // keys: the high word of the modulus (bit 31 marks an available row)
+  unsigned k1[M];
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
+    const cplx f = a[s][j];
+    const double m = fabs(f.re) + fabs(f.im);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(m);
+    const bool ok = (((used[s] >> lane) & 1ull) == 0ull) & (m == m);
+    k1[s] = ok ? ((unsigned)(b >> 32) | 0x80000000u) : 0u;
+  }
+  unsigned km = k1[0];
+#pragma unroll
+  for (int s = 1; s < M; ++s) km = max(km, k1[s]);
+  const unsigned t1 = (unsigned)__builtin_amdgcn_readlane((int)wave_umax(km), 63);
+  int p, pp;
+  if (t1 == 0u) {  // no candidate: keep the row at position kg
+    p = __builtin_amdgcn_readfirstlane(rowat[kg]);
+    pp = kg;
+  } else {
+    unsigned long long e[M];
+    int tot = 0;
+#pragma unroll
+    for (int s = 0; s < M; ++s) {
+      e[s] = __ballot(k1[s] == t1);
+      tot += __builtin_popcountll(e[s]);
+    }
+    if (tot > 1) {  // equal high words: the low words decide, then positions
+      unsigned lo[M];
+      unsigned k2m = 0u;
+#pragma unroll
+      for (int s = 0; s < M; ++s) {
+        const double m = fabs(a[s][j].re) + fabs(a[s][j].im);
+        lo[s] = (unsigned)(unsigned long long)__double_as_longlong(m);
+        k2m = max(k2m, k1[s] == t1 ? lo[s] : 0u);
+      }
+      const unsigned t2 = (unsigned)__builtin_amdgcn_readlane((int)wave_umax(k2m), 63);
+      tot = 0;
+#pragma unroll
+      for (int s = 0; s < M; ++s) {
+        e[s] = __ballot((k1[s] == t1) & (lo[s] == t2));
+        tot += __builtin_popcountll(e[s]);
+      }
+      if (tot > 1) {
+        unsigned k3[M];
+        unsigned k3m = 0u;
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+          const bool tie = ((e[s] >> lane) & 1ull) != 0ull;
+          const int row = lane + 64 * s;
+          k3[s] = tie && row < n ? ~(unsigned)spos[row] : 0u;
+          k3m = max(k3m, k3[s]);
+        }
+        const unsigned t3 = (unsigned)__builtin_amdgcn_readlane((int)wave_umax(k3m), 63);
+#pragma unroll
+        for (int s = 0; s < M; ++s) e[s] = __ballot((k3[s] == t3) & (((e[s] >> lane) & 1ull) != 0ull));
+      }
+    }
+    int sp = 0, lp = 0;
+#pragma unroll
+    for (int s = M - 1; s >= 0; --s)
+      if (e[s] != 0ull) {
+        sp = s;
+        lp = (int)__builtin_ctzll(e[s]);
+      }
+    p = lp + 64 * sp;
+    pp = __builtin_amdgcn_readfirstlane(spos[p]);
+  }
+  // positions: the pivot row takes position kg, the row there the pivot's
+  // old position (issued now; the reciprocal below covers their latency)
+  if (lane == 0) {
+    const int q = rowat[kg];
+    spos[q] = pp;
+    rowat[pp] = q;
+    spos[p] = kg;
+    rowat[kg] = p;
+  }
+  const int sp = p >> 6, lp = p & 63;
+  cplx piv{0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < M; ++s)
+    if (s == sp) piv = cplx{readlane_f64(a[s][j].re, lp), readlane_f64(a[s][j].im, lp)};
+  if (t1 != 0u && piv.re == 0.0 && piv.im == 0.0) singular = true;  // the largest modulus is zero
+  const cplx d = crecip_uniform(piv);
+  if (freed < kGjWaves) {
+    if (!gjf_wait_geq(&cnt[slot], kGjWaves, spin_limit)) {
+      stuck = true;
+      return;
+    }
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+  cplx* gsl = gs + slot * (M * 64);
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
+    cplx g = cmul(a[s][j], d);
+    if (s == sp && lane == lp) g = cplx{0.0, 0.0};
+    gsl[s * 64 + lane] = g;
+    a[s][j] = cplx{-g.re, -g.im};
+    if (s == sp && lane == lp) a[s][j] = cplx{1.0, 0.0};
+  }
+  if (lane == 0) {
+    __hip_atomic_store(&cnt[slot], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    pinfo[slot] = p;
+    dv[k] = d;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  if (lane == 0) __hip_atomic_store(&seq[slot], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ cplx gjf_upd(cplx a, cplx g, cplx P) {
+  return cplx{fma(-g.re, P.re, fma(g.im, P.im, a.re)), fma(-g.re, P.im, fma(-g.im, P.re, a.im))};
+}
+
+template <int M, int R>
+__global__ __launch_bounds__(64 * kGjWaves) void gj_flow_kernel(
+    cplx* A, int n, int lda, long long s_in, long long s_out, int inner, int c0, int wc,
+    int* ipos, int* ipiv, unsigned long long* info, int tag, int spin_limit) {
+  constexpr int NS = GjFlow<M>::NS;
+  __shared__ cplx gs[NS * M * 64];  // multipliers of the published steps
+  __shared__ int pinfo[NS];         // pivot row of each slot's step
+  __shared__ int seq[NS];           // step published in the slot, plus one
+  __shared__ int cnt[NS];           // waves done with the slot's step
+  __shared__ int spos[64 * M];      // position of each row
+  __shared__ int rowat[64 * M];     // row at each position
+  __shared__ cplx dv[kGjWaves * R]; // 1 / pivot of each step
+  const int z = blockIdx.x;
+  const int zo = z / inner, zi = z - zo * inner;
+  A += zi * s_in + zo * s_out;
+  ipos += (long long)z * n;
+  ipiv += (long long)z * n;
+  const int lane = threadIdx.x & 63;
+  const int v = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
+  for (int r = threadIdx.x; r < n; r += 64 * kGjWaves) {
+    const int ps = c0 == 0 ? r : ipos[r];
+    spos[r] = ps;
+    rowat[ps] = r;
+  }
+  if ((int)threadIdx.x < NS) {
+    seq[threadIdx.x] = 0;
+    cnt[threadIdx.x] = kGjWaves;
+  }
+  cplx a[M][R];
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
+    const int row = lane + 64 * s;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int c = v + kGjWaves * j;
+      if (row < n && c < wc) {
+        const double2 t = *reinterpret_cast<const double2*>(A + (long long)row * lda + c0 + c);
+        a[s][j] = cplx{t.x, t.y};
+      } else {
+        a[s][j] = cplx{0.0, 0.0};
+      }
+    }
+  }
+  __syncthreads();
+  unsigned long long used[M];  // rows outside the matrix or already pivots
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
+    const int row = lane + 64 * s;
+    used[s] = __ballot(row >= n || spos[row] < c0);
+  }
+
+  bool singular = false, stuck = false;
+  if (v == 0) {
+    __builtin_amdgcn_s_setprio(3);
+    gjf_produce<M, R, NS>(a, 0, 0, c0, n, lane, used, gs, pinfo, seq, cnt, spos, rowat, dv,
+                          spin_limit, singular, stuck);
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    if (kGjWaves * j >= wc || stuck) break;  // uniform
+    for (int vv = 0; vv < kGjWaves; ++vv) {
+      const int k = kGjWaves * j + vv;
+      if (k >= wc || stuck) break;  // uniform
+      const int slot = k % NS;
+      const bool mine = v == vv;
+      const bool last = vv == kGjWaves - 1;  // step k + 1 is column j + 1 of wave 0
+      const bool prod = k + 1 < wc && v == (last ? 0 : vv + 1);
+      if (prod)
+        __builtin_amdgcn_s_setprio(3);
+      else if (v == (last ? 1 : (vv + 2) % kGjWaves))
+        __builtin_amdgcn_s_setprio(2);
+      else
+        __builtin_amdgcn_s_setprio(0);
+      if (!gjf_wait_geq(&seq[slot], k + 1, spin_limit)) {
+        stuck = true;
+        break;
+      }
+      const int p = __builtin_amdgcn_readfirstlane(pinfo[slot]);
+      const cplx* gsl = gs + slot * (M * 64);
+      cplx g[M];
+#pragma unroll
+      for (int s = 0; s < M; ++s) g[s] = gsl[s * 64 + lane];
+      const int sp = p >> 6, lp = p & 63;
+#pragma unroll
+      for (int s = 0; s < M; ++s)
+        if (s == sp) used[s] |= 1ull << lp;
+      cplx P[R];
+      gj_row<M, R>(a, sp, lp, P);
+      // column j (the owner's was replaced when the step was produced), then
+      // column j + 1, each followed by the next step's production when it is
+      // this wave's; then the other columns
+      if (!mine) {
+#pragma unroll
+        for (int s = 0; s < M; ++s) a[s][j] = gjf_upd(a[s][j], g[s], P[j]);
+      }
+      if (prod && !last)
+        gjf_produce<M, R, NS>(a, j, k + 1, c0, n, lane, used, gs, pinfo, seq, cnt, spos, rowat,
+                              dv, spin_limit, singular, stuck);
+      if (j + 1 < R) {
+#pragma unroll
+        for (int s = 0; s < M; ++s) a[s][j + 1] = gjf_upd(a[s][j + 1], g[s], P[j + 1]);
+        if (prod && last)
+          gjf_produce<M, R, NS>(a, j + 1, k + 1, c0, n, lane, used, gs, pinfo, seq, cnt, spos,
+                                rowat, dv, spin_limit, singular, stuck);
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r == j || r == j + 1) continue;
+#pragma unroll
+        for (int s = 0; s < M; ++s) a[s][r] = gjf_upd(a[s][r], g[s], P[r]);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      if (lane == 0)
+        __hip_atomic_fetch_add(&cnt[slot], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  __builtin_amdgcn_s_setprio(0);
+  __syncthreads();
+
+  // write-back: rows that were pivots of this launch take their 1 / pivot
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
+    const int row = lane + 64 * s;
+    if (row < n) {
+      const int q = spos[row] - c0;
+      const bool scale = q >= 0 && q < wc;
+      const cplx d = scale ? dv[q] : cplx{1.0, 0.0};
+      cplx* dst = A + (long long)row * lda + c0 + v;
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (v + kGjWaves * j < wc) {
+          const cplx o = scale ? cmul(a[s][j], d) : a[s][j];
+          *reinterpret_cast<double2*>(dst + kGjWaves * j) = double2{o.re, o.im};
+        }
+    }
+  }
+  // pivots from the row at each position: a permutation even when a wave
+  // stopped early, so that every later launch indexes rows inside the matrix
+  for (int r = threadIdx.x; r < n; r += 64 * kGjWaves) ipos[r] = spos[r];
+  if ((int)threadIdx.x < wc) ipiv[c0 + threadIdx.x] = rowat[c0 + threadIdx.x];
+  if (lane == 0 && (singular || stuck) && info != nullptr)
+    atomicMin(info, ((unsigned long long)(unsigned)tag << 32) | (unsigned long long)(unsigned)z);
+}
+
 // Panels of blocks above 64 M = 2048 rows: the panel (16 columns) stays in
 // global memory, where it is L2-resident, and is walked 64 rows at a time per
 // step (16 lanes per row, lane j on column j; one workgroup per matrix).  The
@@ -1238,7 +1547,8 @@ __global__ __launch_bounds__(256) void gj_extract_kernel(cplx* A, int lda, long 
   for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total;
        e += (long long)gridDim.x * 256) {
     const int t = (int)(e / N), c = (int)(e - (long long)t * N);
-    double2* src = reinterpret_cast<double2*>(A + (long long)ipiv[t] * lda + c0 + c);
+    const int row = min(max(ipiv[t], 0), n - 1);  // a pivot row; clamped as a guard
+    double2* src = reinterpret_cast<double2*>(A + (long long)row * lda + c0 + c);
     Bz[e] = *src;
     *src = double2{0.0, 0.0};
   }
@@ -1269,10 +1579,12 @@ __global__ __launch_bounds__(256) void gj_permute_kernel(const cplx* S, int lds,
   const int r = blockIdx.x, z = blockIdx.y;
   const int zo = z / inner, zi = z - zo * inner;
   const int* pos = ipos + (long long)z * n;
-  const double2* src = reinterpret_cast<const double2*>(
-      S + zi * ss_in + zo * ss_out + (long long)ipiv[(long long)z * n + r] * lds);
+  // pivots and positions are permutations of 0 .. n - 1; clamped as a guard
+  const int row = min(max(ipiv[(long long)z * n + r], 0), n - 1);
+  const double2* src =
+      reinterpret_cast<const double2*>(S + zi * ss_in + zo * ss_out + (long long)row * lds);
   double2* dst = reinterpret_cast<double2*>(D + zi * sd_in + zo * sd_out + (long long)r * ldd);
-  for (int c = threadIdx.x; c < n; c += 256) dst[c] = src[pos[c]];
+  for (int c = threadIdx.x; c < n; c += 256) dst[c] = src[min(max(pos[c], 0), n - 1)];
 }
 
 // Tall-panel configuration for n rows: M rows per lane (the smallest
@@ -1282,39 +1594,60 @@ __global__ __launch_bounds__(256) void gj_permute_kernel(const cplx* S, int lds,
 struct GjShape {
   int M, R;
 };
-// Panel kernel choice: the per-column kernel (default) or the per-panel one
-// (SINV_GJ_PANEL=panel, or sinv_gj_force_variant); each has its own widest
-// panels that fit its registers without spilling.
-int g_gj_variant = -1;  // SINV_GJ_STEP / SINV_GJ_PANEL; -1 until first read
-bool gj_use_step() {
+// Panel kernel choice: the flow kernel (default, up to 1024 rows; the step
+// kernel above), the per-column step kernel or the per-panel one
+// (SINV_GJ_PANEL=flow / step / panel in the environment, or
+// sinv_gj_force_variant); each has its own widest panels that fit its
+// registers without spilling.
+int g_gj_variant = -1;  // SINV_GJ_FLOW / SINV_GJ_STEP / SINV_GJ_PANEL; -1 until first read
+int gj_variant() {
   if (g_gj_variant < 0) {
     const char* e = getenv("SINV_GJ_PANEL");
-    g_gj_variant = (e != nullptr && strcmp(e, "panel") == 0) ? SINV_GJ_PANEL : SINV_GJ_STEP;
+    int v = SINV_GJ_FLOW;
+    if (e != nullptr && strcmp(e, "panel") == 0) v = SINV_GJ_PANEL;
+    if (e != nullptr && strcmp(e, "step") == 0) v = SINV_GJ_STEP;
+    g_gj_variant = v;
   }
-  return g_gj_variant == SINV_GJ_STEP;
+  return g_gj_variant;
 }
 
-GjShape gj_shape(int n) {
+// The flow kernel holds at most 4 columns per wave from 6 row slots on
+// (registers); at 321 .. 384 rows and 64 or more matrices per launch the
+// step kernel's 64-column panels are taken instead, as the wider panels make
+// the sweep products cheaper than the narrower panels' faster steps save.
+int gj_kind(int n, long long nb) {
+  const int kind = gj_variant();
+  if (kind == SINV_GJ_FLOW && (n + 63) / 64 == 6 && nb >= 64) return SINV_GJ_STEP;
+  return kind;
+}
+
+GjShape gj_shape(int n, long long nb) {
   static constexpr int kMs[] = {2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32};
   const int m = (n + 63) / 64;
-  const bool step = gj_use_step();
+  const int kind = gj_kind(n, nb);
   for (int M : kMs)
     if (M >= m) {
-      if (step) return GjShape{M, M <= 6 ? 8 : (M <= 8 ? 4 : (M <= 14 ? 2 : 1))};
+      if (kind == SINV_GJ_FLOW && M <= kGjFlowMaxM)
+        return GjShape{M, M <= 5 ? 8 : (M <= 8 ? 4 : (M <= 12 ? 2 : 1))};
+      if (kind != SINV_GJ_PANEL)
+        return GjShape{M, M <= 6 ? 8 : (M <= 8 ? 4 : (M <= 14 ? 2 : 1))};
       return GjShape{M, M <= 2 ? 8 : (M <= 4 ? 4 : (M <= 8 ? 2 : 1))};
     }
   return GjShape{0, 0};
 }
-int gj_width(int n) {
-  const GjShape g = gj_shape(n);
+int gj_width(int n, long long nb) {
+  const GjShape g = gj_shape(n, nb);
   return g.M > 0 ? kGjWaves * g.R : kGjGlobalW;
 }
 
-template <int M, int R, bool STEP>
+template <int M, int R, int KIND>
 void gj_panel_mr(cplx* A, int n, int lda, long long s_in, long long s_out, int inner, unsigned nb,
                  int c0, int wc, int* ipos, int* ipiv, unsigned long long* info, int tag,
                  hipStream_t s) {
-  if constexpr (STEP)
+  if constexpr (KIND == SINV_GJ_FLOW && M <= kGjFlowMaxM)
+    hipLaunchKernelGGL((gj_flow_kernel<M, R>), dim3(nb), dim3(64 * kGjWaves), 0, s, A, n, lda,
+                       s_in, s_out, inner, c0, wc, ipos, ipiv, info, tag, g_leaf_spin_limit);
+  else if constexpr (KIND != SINV_GJ_PANEL)
     hipLaunchKernelGGL((gj_step_kernel<M, R>), dim3(nb), dim3(64 * kGjWaves), 0, s, A, n, lda,
                        s_in, s_out, inner, c0, wc, ipos, ipiv, info, tag);
   else
@@ -1327,56 +1660,80 @@ hipError_t launch_gj_panel(cplx* A, int n, int lda, long long s_in, long long s_
                            unsigned long long* info, int tag, hipStream_t s) {
   const unsigned nb = (unsigned)(inner * outer);
   const int pid = prof_open(s);
-#define SINV_GJ_CASE(M_, R_, STEP_)                                                     \
+#define SINV_GJ_CASE(M_, R_, KIND_)                                                     \
   case M_ * 100 + R_:                                                                  \
-    gj_panel_mr<M_, R_, STEP_>(A, n, lda, s_in, s_out, inner, nb, c0, wc, ipos, ipiv,   \
+    gj_panel_mr<M_, R_, KIND_>(A, n, lda, s_in, s_out, inner, nb, c0, wc, ipos, ipiv,   \
                                info, tag, s);                                          \
     break;
-  const GjShape gs = gj_shape(n);
+#define SINV_GJ_CASES(KIND_)      \
+  SINV_GJ_CASE(2, 8, KIND_)       \
+  SINV_GJ_CASE(3, 8, KIND_)       \
+  SINV_GJ_CASE(4, 8, KIND_)       \
+  SINV_GJ_CASE(5, 8, KIND_)       \
+  SINV_GJ_CASE(6, 8, KIND_)       \
+  SINV_GJ_CASE(7, 4, KIND_)       \
+  SINV_GJ_CASE(8, 4, KIND_)       \
+  SINV_GJ_CASE(10, 2, KIND_)      \
+  SINV_GJ_CASE(12, 2, KIND_)      \
+  SINV_GJ_CASE(14, 2, KIND_)      \
+  SINV_GJ_CASE(16, 1, KIND_)      \
+  SINV_GJ_CASE(20, 1, KIND_)      \
+  SINV_GJ_CASE(24, 1, KIND_)      \
+  SINV_GJ_CASE(28, 1, KIND_)      \
+  SINV_GJ_CASE(32, 1, KIND_)
+  const GjShape gs = gj_shape(n, nb);
+  const int kind = gj_kind(n, nb);
   if (gs.M == 0) {
     hipLaunchKernelGGL(gj_panel_global_kernel, dim3(nb), dim3(1024), 2 * n * sizeof(int), s, A,
                        n, lda, s_in, s_out, inner, c0, wc, ipos, ipiv, info, tag);
-  } else if (gj_use_step()) {
+  } else if (kind == SINV_GJ_FLOW) {
     switch (gs.M * 100 + gs.R) {
-      SINV_GJ_CASE(2, 8, true)
-      SINV_GJ_CASE(3, 8, true)
-      SINV_GJ_CASE(4, 8, true)
-      SINV_GJ_CASE(5, 8, true)
-      SINV_GJ_CASE(6, 8, true)
-      SINV_GJ_CASE(7, 4, true)
-      SINV_GJ_CASE(8, 4, true)
-      SINV_GJ_CASE(10, 2, true)
-      SINV_GJ_CASE(12, 2, true)
-      SINV_GJ_CASE(14, 2, true)
-      SINV_GJ_CASE(16, 1, true)
-      SINV_GJ_CASE(20, 1, true)
-      SINV_GJ_CASE(24, 1, true)
-      SINV_GJ_CASE(28, 1, true)
-      SINV_GJ_CASE(32, 1, true)
+      SINV_GJ_CASE(2, 8, SINV_GJ_FLOW)
+      SINV_GJ_CASE(3, 8, SINV_GJ_FLOW)
+      SINV_GJ_CASE(4, 8, SINV_GJ_FLOW)
+      SINV_GJ_CASE(5, 8, SINV_GJ_FLOW)
+      SINV_GJ_CASE(6, 4, SINV_GJ_FLOW)
+      SINV_GJ_CASE(7, 4, SINV_GJ_FLOW)
+      SINV_GJ_CASE(8, 4, SINV_GJ_FLOW)
+      SINV_GJ_CASE(10, 2, SINV_GJ_FLOW)
+      SINV_GJ_CASE(12, 2, SINV_GJ_FLOW)
+      SINV_GJ_CASE(14, 1, SINV_GJ_FLOW)
+      SINV_GJ_CASE(16, 1, SINV_GJ_FLOW)
+      SINV_GJ_CASE(20, 1, SINV_GJ_FLOW)
+      SINV_GJ_CASE(24, 1, SINV_GJ_FLOW)
+      SINV_GJ_CASE(28, 1, SINV_GJ_FLOW)
+      SINV_GJ_CASE(32, 1, SINV_GJ_FLOW)
+      default:
+        return hipErrorInvalidValue;
+    }
+  } else if (kind == SINV_GJ_STEP) {
+    switch (gs.M * 100 + gs.R) {
+      SINV_GJ_CASES(SINV_GJ_STEP)
       default:
         return hipErrorInvalidValue;
     }
   } else {
     switch (gs.M * 100 + gs.R) {
-      SINV_GJ_CASE(2, 8, false)
-      SINV_GJ_CASE(3, 4, false)
-      SINV_GJ_CASE(4, 4, false)
-      SINV_GJ_CASE(5, 2, false)
-      SINV_GJ_CASE(6, 2, false)
-      SINV_GJ_CASE(7, 2, false)
-      SINV_GJ_CASE(8, 2, false)
-      SINV_GJ_CASE(10, 1, false)
-      SINV_GJ_CASE(12, 1, false)
-      SINV_GJ_CASE(14, 1, false)
-      SINV_GJ_CASE(16, 1, false)
-      SINV_GJ_CASE(20, 1, false)
-      SINV_GJ_CASE(24, 1, false)
-      SINV_GJ_CASE(28, 1, false)
-      SINV_GJ_CASE(32, 1, false)
+      SINV_GJ_CASE(2, 8, SINV_GJ_PANEL)
+      SINV_GJ_CASE(3, 4, SINV_GJ_PANEL)
+      SINV_GJ_CASE(4, 4, SINV_GJ_PANEL)
+      SINV_GJ_CASE(5, 2, SINV_GJ_PANEL)
+      SINV_GJ_CASE(6, 2, SINV_GJ_PANEL)
+      SINV_GJ_CASE(7, 2, SINV_GJ_PANEL)
+      SINV_GJ_CASE(8, 2, SINV_GJ_PANEL)
+      SINV_GJ_CASE(10, 1, SINV_GJ_PANEL)
+      SINV_GJ_CASE(12, 1, SINV_GJ_PANEL)
+      SINV_GJ_CASE(14, 1, SINV_GJ_PANEL)
+      SINV_GJ_CASE(16, 1, SINV_GJ_PANEL)
+      SINV_GJ_CASE(20, 1, SINV_GJ_PANEL)
+      SINV_GJ_CASE(24, 1, SINV_GJ_PANEL)
+      SINV_GJ_CASE(28, 1, SINV_GJ_PANEL)
+      SINV_GJ_CASE(32, 1, SINV_GJ_PANEL)
       default:
         return hipErrorInvalidValue;
     }
   }
+#undef SINV_GJ_CASES
 #undef SINV_GJ_CASE
   prof_close(pid, s, PROF_INV_PANEL, 8.0 * n * (double)wc * wc * nb,
              2.0 * sizeof(cplx) * n * (double)wc * nb);
@@ -1480,7 +1837,7 @@ hipError_t gj_eliminate(cplx* A, int n, int lda, long long s_in, long long s_out
   const long long nb = (long long)inner * outer;
   *ipos = reinterpret_cast<int*>(work + (long long)n * n * nb);
   *ipiv = *ipos + (long long)n * nb;
-  return gj_range(A, n, lda, s_in, s_out, inner, outer, 0, n, gj_width(n), work, *ipos, *ipiv,
+  return gj_range(A, n, lda, s_in, s_out, inner, outer, 0, n, gj_width(n, nb), work, *ipos, *ipiv,
                   info, tag, s);
 }
 
@@ -1574,7 +1931,7 @@ extern "C" int sinv_leaf_force_variant(int variant) {
 
 extern "C" int sinv_gj_force_variant(int variant) {
   using namespace sinv;
-  if (variant != SINV_GJ_STEP && variant != SINV_GJ_PANEL) {
+  if (variant != SINV_GJ_STEP && variant != SINV_GJ_PANEL && variant != SINV_GJ_FLOW) {
     set_error("sinv_gj_force_variant: unknown variant %d", variant);
     return SINV_E_INVAL;
   }

@@ -149,6 +149,8 @@ def test_int8x_host_validation(built_lib):
     assert rc == nat.SINV_E_INVAL and b"exceeds" in L.sinv_last_error()
     assert L.sinv_int8x_force_variant(2) == nat.SINV_E_INVAL
     assert L.sinv_int8x_force_variant(-1) == nat.SINV_OK
+    assert L.sinv_gj_force_variant(3) == nat.SINV_E_INVAL
+    assert L.sinv_gj_force_variant(2) == nat.SINV_OK
     d = nat.RgfDesc(nblocks=8, blocksize=96, batch=4, precision=0, diag=1, upper=1, lower=1,
                     g_diag=1, g_upper=1, g_lower=1, save_off_diag=1)
     w64 = L.sinv_rgf_workspace_bytes(ctypes.byref(d))

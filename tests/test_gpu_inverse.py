@@ -130,11 +130,12 @@ def test_zinv_singular_reported(built_lib, n):
     assert ei.value.instance == 1
 
 
-@pytest.mark.parametrize("n", [100, 256, 384, 416, 512, 1100])
-def test_zinv_both_panel_kernels(built_lib, n):
-    """The per-column and per-panel tall-panel kernels (different panel widths,
-    so different sweep shapes) both give numpy's inverse, pivoting over all
-    rows, and agree with each other to rounding."""
+@pytest.mark.parametrize("n", [100, 256, 384, 416, 512, 700, 1000, 1100])
+def test_zinv_all_panel_kernels(built_lib, n):
+    """The flow, per-column and per-panel tall-panel kernels (different panel
+    widths and update orders, so different sweep shapes and rounding) all give
+    numpy's inverse, pivoting over all rows, and agree with each other to
+    rounding."""
     from sinv_amd import _native as nat, core
     L = nat.lib()
     rng = np.random.default_rng(77 + n)
@@ -142,13 +143,32 @@ def test_zinv_both_panel_kernels(built_lib, n):
     A[1, 0, :] *= 1e-3
     out = {}
     try:
-        for v in (1, 0):   # SINV_GJ_PANEL, SINV_GJ_STEP
+        for v in (1, 0, 2):   # SINV_GJ_PANEL, SINV_GJ_STEP, SINV_GJ_FLOW
             assert L.sinv_gj_force_variant(v) == 0
             out[v] = core.zinv_batched(A).cpu().numpy()
     finally:
-        L.sinv_gj_force_variant(0)
+        L.sinv_gj_force_variant(2)
     for z in range(2):
-        for v in (0, 1):
+        for v in (0, 1, 2):
             _check(out[v][z], A[z], (n, z, v))
-        assert np.linalg.norm(out[0][z] - out[1][z]) / np.linalg.norm(out[1][z]) < \
-            1e-15 * np.linalg.cond(A[z]) * n
+        for v in (0, 2):
+            assert np.linalg.norm(out[v][z] - out[1][z]) / np.linalg.norm(out[1][z]) < \
+                1e-15 * np.linalg.cond(A[z]) * n
+
+
+def test_zinv_flow_wait_bound_reports(built_lib):
+    """A flow-kernel wave whose hand-off wait runs out stops and reports its
+    matrix through info (forced from the host with a zero poll budget), so the
+    kernel exits; the default budget restores normal results."""
+    from sinv_amd import _native as nat, core
+    L = nat.lib()
+    rng = np.random.default_rng(5)
+    A = _rand(rng, 2, 200, 200)
+    try:
+        assert L.sinv_gj_force_variant(2) == 0
+        assert L.sinv_leaf_spin_limit(0) == 0
+        with pytest.raises(np.linalg.LinAlgError):
+            core.zinv_batched(A)
+    finally:
+        L.sinv_leaf_spin_limit(-1)
+    _check(core.zinv_batched(A).cpu().numpy()[1], A[1], "after")
