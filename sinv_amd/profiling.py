@@ -25,13 +25,15 @@ from collections import defaultdict
 
 
 def _short(name):
+    name = name.replace("(anonymous namespace)::", "")
     # every tile configuration of the GEMM counts as one kernel per precision
     for key, short in (("gemm_kernel<double", "gemm_kernel<double>"),
                        ("gemm_kernel<float", "gemm_kernel<float>")):
         if key in name:
             return short
-    for key in ("inv_leaf_kernel", "axpby_shift_kernel", "transpose_kernel", "fill_u64_kernel",
-                "fill_kernel"):
+    for key in ("gemm_i8x", "slice_piece_kernel", "piece_max_kernel", "inv_leaf_kernel", "gj_flow_kernel", "gj_step_kernel", "gj_panel_global_kernel",
+                "gj_panel_kernel", "gj_extract_kernel", "gj_copy_kernel", "gj_permute_kernel",
+                "axpby_shift_kernel", "transpose_kernel", "fill_u64_kernel", "fill_kernel"):
         if key in name:
             return key
     return name.split("(")[0][-60:]
