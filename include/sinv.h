@@ -19,8 +19,8 @@
  *     SINV_E_HIP (a HIP runtime error; text via sinv_last_error()).
  *     Singular pivots are reported through `info` (device), not the return
  *     code, because the call does not synchronise: info holds
- *     (block << 32 | instance) of the first singular leaf pivot, or
- *     SINV_INFO_OK.
+ *     (block << 32 | instance) of the first block whose elimination met a
+ *     zero pivot (or whose hand-off wait ran out), or SINV_INFO_OK.
  */
 #ifndef SINV_H_
 #define SINV_H_
@@ -278,9 +278,13 @@ int sinv_pdiv_batched(const sinv_pdiv_desc* d, void* work, size_t work_bytes, vo
 /* ------------------------------------------------------------------------
  * Batched in-place complex128 block inverse (kernel K1).
  * Replaces the numpy.linalg.inv calls of the hot path (rgf.py:44,48;
- * bcr_serial.py:121,148,249,302,379).  Blocks with n <= 64 use Gauss-Jordan
- * with partial pivoting on |re| + |im| (LAPACK ?gesv's modulus); larger blocks
- * use a 2x2 Schur-complement recursion down to such leaves.
+ * bcr_serial.py:121,148,249,302,379).  Gauss-Jordan elimination with partial
+ * pivoting over all n rows on |re| + |im| (the modulus of LAPACK's izamax,
+ * which numpy.linalg.inv's zgetrf uses): one workgroup per block for n <= 64;
+ * above, blocked by columns, tall panels eliminated over all rows by one
+ * workgroup each and their effect on the other columns applied by batched
+ * FP64 MFMA products (n <= 8192; the workspace holds one n x n copy per
+ * block plus row positions and pivots).
  * Matrix z is at A + z*strideA (complex elements), leading dimension lda.
  * ------------------------------------------------------------------------ */
 size_t sinv_zinv_workspace_bytes(int n, int batch);
@@ -428,7 +432,8 @@ int sinv_gemm_select(int M, int N, int K, long long batch, int opA, int opB, int
  * Between sinv_prof_begin and sinv_prof_end every kernel launch of the GEMM
  * and leaf-inverse kernels is bracketed by hipEvents on its own stream.
  * sinv_prof_end synchronises those events and returns, per kind
- * (0 = FP64 GEMM, 1 = FP32 GEMM, 2 = leaf inverse, 3 = other), the summed
+ * (0 = FP64 GEMM, 1 = FP32 GEMM, 2 = leaf inverse, 3 = other, 4 = int8x
+ * GEMM call, 5 = tall-panel elimination of the blocked inverse), the summed
  * kernel time in ms, the launch count, the algorithmic flops (8 M N K per
  * complex GEMM, 8 n^3 per inverse) and, when `bytes` is not NULL, the
  * algorithmic bytes (16 B per complex element of each operand read and each

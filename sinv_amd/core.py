@@ -113,6 +113,20 @@ def rgf_batched(diag, upper, lower, *, sym_mat=False, save_off_diag=True, precis
     Returns (G_diag (B,N,b,b), G_upper (B,N-1,b,b), G_lower (B,N-1,b,b)).
     Follows rgf.py:40-62; with save_off_diag=False the off-diagonals are NaN
     (rgf.py:41,55), with sym_mat=True G_{i,i+1} = G_{i+1,i}^T (rgf.py:58).
+
+    precision (the same choices for every RGF entry point; the adjoint takes
+    all but "mixed_refined"):
+      "fp64"           complex128 throughout, the reference's arithmetic;
+      "int8x"          complex128 products formed from 7 int8 digit planes on the
+                       int8 matrix cores, FP64 inverses: within 1e-9 of "fp64"
+                       on the benchmark instances;
+      "mixed"          complex64 products, FP64 inverses: about 1e-7 relative
+                       rounding per product, amplified by the conditioning of
+                       the Schur complements -- errors of 1e-3 relative near
+                       the band at eta = 1e-3 are measured;
+      "mixed_refined"  complex64 forward products, one FP64 refinement step of
+                       each g_i, FP64 backward sweep: up to ~1e-7 on the same
+                       instances.
     """
     if precision not in nat.PREC:
         raise ValueError(f"precision must be one of {sorted(nat.PREC)}")

@@ -13,9 +13,11 @@
 // partial pivoting over all n rows (exact first maximum of |re| + |im| in
 // position order, as izamax), blocked by columns: tall panels of 64, 32, 16
 // or 8 columns are eliminated over all rows by a register-resident kernel
-// (an L2-resident one above 2048 rows), and each panel's elimination reaches
-// the other columns through batched GEMMs on FP64 MFMA.  Cost 8 n^3 real
-// flops either way.
+// (gj_flow_kernel up to 1024 rows, gj_step_kernel above and for 321 .. 384
+// rows in launches of 64 or more blocks, the L2-resident
+// gj_panel_global_kernel above 2048 rows), and each panel's elimination
+// reaches the other columns through batched GEMMs on FP64 MFMA.  Cost 8 n^3
+// real flops either way.
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
