@@ -55,6 +55,8 @@ def parse():
                     choices=["energies", "dominant", "large", "adjoint", "convert", "variants",
                              "gemm"])
     ap.add_argument("--chunk", type=int, default=128, help="energies per adjoint launch")
+    ap.add_argument("--eager", action="store_true",
+                    help="time eager launches instead of replays of one step captured into a HIP graph")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward+adjoint record of the default run")
@@ -211,8 +213,16 @@ def main():
     def timed(plans, steps, warmup, tag):
         # the block-size groups are independent: run them concurrently, one stream each
         pset = core.PlanSet(plans)
+        run = pset.run
+        if not args.eager:
+            # one step captured into a HIP graph (the fork / join across the
+            # group streams recorded as dependencies) and replayed
+            graph = pset.capture(warmup=max(warmup, 1))
+            torch.cuda.synchronize()
+            pset.check()
+            run = graph.replay
         for _ in range(warmup):
-            pset.run()
+            run()
         torch.cuda.synchronize()
         pset.check()
         # roctx range: lets a rocprofv3 --marker-trace run select this region's dispatches
@@ -222,7 +232,7 @@ def main():
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
         for _ in range(steps):
-            pset.run()
+            run()
         ev1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
@@ -291,7 +301,7 @@ def main():
         f"int8x {err_i8x:.3e}")
 
     # ------------------------------------------------------------ FP64 vs the CPU oracle (sample)
-    err_fp64 = err_floor = None
+    err_fp64 = err_floor = err_sample = None
     if not args.no_cpu:
         from oracle import rgf_ref
         g = groups[0]
@@ -311,6 +321,9 @@ def main():
         # triple product associated the other way (an equally valid FP64 order)
         alt = rgf_ref.rgf_blocks(d, u, l, reassociate=True)
         err_floor = float(np.sqrt(sum(np.linalg.norm(a - r) ** 2 for a, r in zip(alt, ref)) / den))
+        err_sample = (f"b = {g['b']} group, {'first energy' if g['shift'] is not None else 'the instance'}: "
+                      "relative Frobenius distance over every selected block; the floor is the oracle "
+                      "against itself with each triple product associated the other way")
         log(f"[bench] fp64 GPU vs CPU oracle (b={g['b']}, first energy): {err_fp64:.3e} "
             f"(oracle vs its reassociated self: {err_floor:.3e})")
 
@@ -325,20 +338,35 @@ def main():
                                dscale=g["dscale"], oscale=g["oscale"]) for k in range(min(nsamp, nE))]
             ps[0].run()
             torch.cuda.synchronize()
+
+            def run_ps():
+                for p in ps:
+                    p.run()
+            if not args.eager:  # the same launch mode as the batched steps
+                side = torch.cuda.Stream()
+                side.wait_stream(torch.cuda.current_stream())
+                with torch.cuda.stream(side):
+                    run_ps()
+                torch.cuda.current_stream().wait_stream(side)
+                gp = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(gp):
+                    run_ps()
+                torch.cuda.synchronize()
+                run_ps = gp.replay
             ev0 = torch.cuda.Event(enable_timing=True)
             ev1 = torch.cuda.Event(enable_timing=True)
             ev0.record(stream)
-            for p in ps:
-                p.run()
+            run_ps()
             ev1.record(stream)
             torch.cuda.synchronize()
             t = ev0.elapsed_time(ev1) / len(ps)
             per_b.append((g["b"], nE, t))
             log(f"[bench] plain fp64 batch=1 b={g['b']}: {t:.2f} ms/solve")
-            del ps
+            del ps, run_ps
         step_ms = sum(nE * t for _, nE, t in per_b)
         plain = dict(solves_per_s=solves / (step_ms / 1e3), ms_per_solve={str(b): t for b, _, t in per_b},
-                     sample=f"{nsamp} energies per block size, batch=1 launch sequences")
+                     sample=f"{nsamp} energies per block size, batch=1 launch sequences",
+                     launch="eager" if args.eager else "hip graph replay")
 
     # ------------------------------------------------------------ CPU baseline (oracle, sampled)
     # numpy restatement of rgf.py (the reference's own inv / @ calls), on the
@@ -548,16 +576,14 @@ def main():
                   "mixed_refined": "complex64-schur-gemm/complex128-refinement",
                   "int8x": "int8-digit-gemm/complex128"}[head],
         "data": "synthetic",
-        "config": wl,
+        "config": dict(wl, launch="eager" if args.eager else
+                       "hip graph replay of one captured step (groups on their own streams)"),
         "tflops": round(alg_flops / (ms / 1e3) / 1e12, 3),
         "headline_mode": head,
         "modes": {
             "batched_fp64": dict(solves_per_s=round(fp64_sps, 3), ms_per_step=round(results["fp64"]["ms"], 3),
                                  err_vs_cpu_oracle=err_fp64, oracle_rounding_floor=err_floor,
-                                 err_sample="b = 128 group, first energy: relative Frobenius "
-                                            "distance over every selected block; the floor is the "
-                                            "oracle against itself with each triple product "
-                                            "associated the other way"),
+                                 err_sample=err_sample),
             "batched_mixed": dict(solves_per_s=round(mixed_sps, 3), ms_per_step=round(results["mixed"]["ms"], 3),
                                   max_rel_err_vs_fp64=err_mixed, passes_err_gate=err_mixed <= ERR_GATE),
             "batched_mixed_refined": dict(
