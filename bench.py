@@ -608,8 +608,8 @@ def main():
                                 if v[0] > 0 else 0.0) for k, v in results[head]["prof"].items() if v[1]},
     }
     if args.workload == "energies" and not args.no_adjoint:
-        # the forward+adjoint workload (FP64, at most two timed steps after one
-        # warm-up, so the default run stays within minutes) as a second record
+        # the forward+adjoint workload (every mode, at most two timed steps after
+        # one warm-up, so the default run stays within minutes) as a second record
         # the headline's outputs and inputs are no longer needed: free them so
         # the second workload has the device to itself
         for g in groups:
@@ -620,7 +620,8 @@ def main():
         torch.cuda.empty_cache()
         log(f"[bench] forward+adjoint workload (second record); "
             f"{torch.cuda.memory_allocated() / 1e9:.1f} GB still allocated")
-        out["forward_adjoint"] = adjoint_record(args, modes=("fp64",), steps=min(args.steps, 2),
+        out["forward_adjoint"] = adjoint_record(args, modes=("fp64", "mixed", "int8x"),
+                                                steps=min(args.steps, 2),
                                                 warmup=min(args.warmup, 1),
                                                 cpu_baseline=not args.no_cpu)
     print(json.dumps(out), flush=True)
