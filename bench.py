@@ -665,11 +665,11 @@ def adjoint_record(args, modes=("fp64", "mixed", "int8x"), steps=None, warmup=No
             Ad, Au, Al, Gd = core.rgf_adjoint_ragged(Hd, Hu, Hl, gd, zu, zl, shift=sh, dscale=-1.0,
                                                      oscale=-1.0, precision=prec,
                                                      checkpoint_every=every, check=False,
-                                                     return_g_diag=True)
+                                                     return_g_diag=True, sum_batch=True)
             for g in Gd:
                 loss -= torch.diagonal(g, dim1=-2, dim2=-1).imag.sum() / np.pi
             for acc, a in zip(grad, Ad + Au + Al):
-                acc -= a.sum(0)          # A = (E + i eta) I - H: dL/dH = -sum_E dL/dA
+                acc -= a                 # A = (E + i eta) I - H: dL/dH = -sum_E dL/dA
             del Gd, Ad, Au, Al
         return loss, grad
 
@@ -722,10 +722,13 @@ def adjoint_record(args, modes=("fp64", "mixed", "int8x"), steps=None, warmup=No
         peak = (torch.cuda.max_memory_allocated() - base_mem) / 1e9
         log(f"[bench] {prec}: {ms:.1f} ms/step, {nE / (ms / 1e3):.1f} solves/s (fwd+adj), "
             f"peak {peak:.1f} GB")
-        # kernel-level figures from one energy chunk (same shapes as every chunk)
+        # kernel-level figures from one energy chunk (same shapes as every chunk),
+        # inside a roctx range that a rocprofv3 --marker-trace run can select
+        torch.cuda.nvtx.range_push(f"sinv_bench_profiled_adjoint:{prec}")
         with _native.LaunchProfiler(max_launches=100000) as prof:
             step(prec, limit=args.chunk)
             torch.cuda.synchronize()
+        torch.cuda.nvtx.range_pop()
         out[prec] = dict(ms=ms, loss=float(loss), grad=grad, prof=prof.stats, bytes=prof.bytes,
                          peak_gb=peak)
 

@@ -467,9 +467,12 @@ def rgf_ragged(diag, upper, lower, *, sym_mat=False, save_off_diag=True, precisi
 
 def rgf_adjoint_ragged(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *, shift=None,
                        dscale=1.0, oscale=1.0, precision="fp64", checkpoint_every=0, check=True,
-                       return_g_diag=False):
+                       return_g_diag=False, sum_batch=False):
     """``rgf_adjoint_batched`` with per-block sizes (lists of blocks, as
-    ``rgf_ragged``).  Returns lists of gradient blocks (B, ...)."""
+    ``rgf_ragged``).  Returns lists of gradient blocks (B, ...); with
+    ``sum_batch=True`` the gradient blocks are summed over the batch (one
+    reduction over each packed output, e.g. dL/dA summed over energies) and
+    have no batch dimension (G_ii, when returned, keeps it)."""
     if precision not in nat.PREC:
         raise ValueError(f"precision must be one of {sorted(nat.PREC)}")
     bs = _ragged_sizes(diag, upper, lower)
@@ -508,9 +511,13 @@ def rgf_adjoint_ragged(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *,
               "rgf_adjoint")
     if check:
         nat.check_info(info)
+    if sum_batch:
+        Ad, Au, Al = (x.sum(0, keepdim=True) for x in (Ad, Au, Al))
     out = (_unpack(Ad, [(b, b) for b in bs]),
            _unpack(Au, [(bs[i], bs[i + 1]) for i in range(N - 1)]),
            _unpack(Al, [(bs[i + 1], bs[i]) for i in range(N - 1)]))
+    if sum_batch:
+        out = tuple([x[0] for x in part] for part in out)
     if return_g_diag:
         return out + (_unpack(Gd, [(b, b) for b in bs]),)
     return out

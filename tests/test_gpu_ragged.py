@@ -62,3 +62,28 @@ def test_adjoint_ragged_vs_dense(built_lib, every):
     gG = dense.assemble_lists(*g)
     R = dense.select_lists(-G.conj().T @ gG @ G.conj().T, bs)
     assert dense.rel_err_lists(got, R) < 1e-12
+
+
+def test_adjoint_ragged_sum_batch(built_lib):
+    """sum_batch=True returns the gradient blocks summed over the batch (the
+    energies' dL/dA added up), equal to summing the per-instance blocks, with
+    G_ii still per instance."""
+    import torch
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    bs = [40, 72, 24, 65]
+    H = configs.hermitian_H_ragged(bs, seed=5)
+    E = configs.energies(5, seed=6) + 1e-2j
+    gd = [(-1j / np.pi) * np.eye(b) for b in bs]
+    zu = [np.zeros((bs[i], bs[i + 1]), complex) for i in range(len(bs) - 1)]
+    zl = [np.zeros((bs[i + 1], bs[i]), complex) for i in range(len(bs) - 1)]
+    kw = dict(shift=E, dscale=-1.0, oscale=-1.0, checkpoint_every=2, return_g_diag=True)
+    per = core.rgf_adjoint_ragged(*H, gd, zu, zl, **kw)
+    tot = core.rgf_adjoint_ragged(*H, gd, zu, zl, sum_batch=True, **kw)
+    for part_per, part_tot in zip(per[:3], tot[:3]):
+        for a, s in zip(part_per, part_tot):
+            assert s.shape == a.shape[1:]
+            ref = a.sum(0)
+            assert float((s - ref).abs().max()) <= 1e-13 * max(float(ref.abs().max()), 1.0)
+    for a, s in zip(per[3], tot[3]):
+        assert torch.equal(a, s)
