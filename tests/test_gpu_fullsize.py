@@ -6,7 +6,9 @@ do not need a dense or CPU reference there:
   column i, G_{i,i-1} A_{i-1,i} + G_ii A_ii + G_{i,i+1} A_{i+1,i} = I;
 * RGF and BCR, two different eliminations, give the same blocks;
 * for A(E) = (E + i eta) I - H with H Hermitian and eta > 0 the density of
-  states -Im tr G_ii / pi is non-negative for every block.
+  states -Im tr G_ii / pi is non-negative for every block;
+* sampled columns of A^-1 from LAPACK's banded LU (an independent pivoted
+  solver that fits the host at n = 65536) match the selected blocks.
 
 Sizes: the single large instance (128 blocks of 512, diagonally dominant
 generator) and the batched-energies workload's largest block size (64
@@ -88,6 +90,53 @@ def test_single_large_instance_rgf_and_bcr(built_lib):
         rn, cn = _residuals(d, u, l, *G)
         assert rn < 1e-12 and cn < 1e-12, (rn, cn)
     assert _rel(G_bcr, G_rgf) < 1e-12
+
+
+def test_single_large_instance_against_banded_columns(built_lib):
+    """128 blocks of 512 (n = 65536): columns of A^-1 from an independent
+    pivoted solver -- LAPACK's banded LU (zgbsv through
+    scipy.linalg.solve_banded, bandwidths 2b - 1) on A e_j -- in a first, a
+    middle and a last block column; the GPU's selected blocks in those columns
+    (block rows c - 1, c, c + 1) must match them to 1e-12 relative."""
+    import scipy.linalg as sl
+    import torch
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    print("full size: 128 blocks of 512, rgf vs banded-LU columns", flush=True)
+    N, b = 128, 512
+    dh, uh, lh = configs.dominant_blocks(N, b, seed=2)
+    d, u, l = (core.to_dev(x)[None] for x in (dh, uh, lh))
+    Gd, Gu, Gl = (x[0] for x in core.rgf_batched(d, u, l))
+    torch.cuda.synchronize()
+    n, kl = N * b, 2 * b - 1
+    ab = np.zeros((2 * kl + 1, n), complex)   # ab[kl + r - c, c] = A[r, c]
+    cols = np.arange(b)[None, :]
+    rows = np.arange(b)[:, None]
+    for i in range(N):
+        for j, blk in ((i, dh[i]), (i + 1, uh[i] if i + 1 < N else None),
+                       (i - 1, lh[i - 1] if i >= 1 else None)):
+            if blk is None:
+                continue
+            r, c = i * b + rows, j * b + cols
+            ab[kl + r - c, np.broadcast_to(c, (b, b))] = blk
+    picks = [(0, 3), (N // 2, 257), (N - 1, 510)]     # (block column, column inside it)
+    rhs = np.zeros((n, len(picks)), complex)
+    for q, (c, k) in enumerate(picks):
+        rhs[c * b + k, q] = 1.0
+    X = sl.solve_banded((kl, kl), ab, rhs, check_finite=False)
+    num = den = 0.0
+    for q, (c, k) in enumerate(picks):
+        for r, blk in ((c, Gd[c]), (c - 1, Gu[c - 1] if c >= 1 else None),
+                       (c + 1, Gl[c] if c + 1 < N else None)):
+            if blk is None:
+                continue
+            got = blk[:, k].cpu().numpy()
+            ref = X[r * b:(r + 1) * b, q]
+            num += np.linalg.norm(got - ref) ** 2
+            den += np.linalg.norm(ref) ** 2
+    err = np.sqrt(num / den)
+    print(f"selected blocks vs banded-LU columns: {err:.3e}", flush=True)
+    assert err < 1e-12
 
 
 def test_energies_full_blocks_residuals_and_dos(built_lib):
