@@ -65,6 +65,10 @@ def parse():
                     help="skip BCR-serial on the single-instance workloads")
     ap.add_argument("--no-pcie", action="store_true",
                     help="skip the PCIe-inclusive figure (pinned copy rates, drop-in call)")
+    ap.add_argument("--inverse", default="auto", choices=["auto", "pivoted"],
+                    help="block inverse of the RGF plans: 'auto' takes the Newton-Schulz "
+                         "iteration for latency-bound calls (at most 4 instances) on column "
+                         "diagonally dominant matrices, the pivoted elimination otherwise")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the GEMM tile autotuner (untimed, before the timed steps)")
     ap.add_argument("--save-tune", default=None, help="write the tuned GEMM table (JSON) here")
@@ -175,9 +179,11 @@ def main():
         for g in groups:
             sh = g["shift"] if energies_slice is None or g["shift"] is None else g["shift"][energies_slice]
             p = core.RgfPlan(*g["dev"], shift=sh, dscale=g["dscale"], oscale=g["oscale"],
-                             precision=precision, out=g["out"] if energies_slice is None else None)
+                             precision=precision, out=g["out"] if energies_slice is None else None,
+                             inverse=args.inverse)
             if energies_slice is None:
                 g["out"] = p.outputs()
+                g["inverse"] = p.inverse[0]
             plans.append(p)
         return plans
 
@@ -289,7 +295,7 @@ def main():
             outs = []
             for prec in ("fp64", "mixed", "mixed_refined", "int8x"):
                 p = core.RgfPlan(*g["dev"], shift=sh, dscale=g["dscale"], oscale=g["oscale"],
-                                 precision=prec, batch=None)
+                                 precision=prec, batch=None, inverse=args.inverse)
                 p.run()
                 outs.append(p.outputs())
             err_mixed = max(err_mixed, rel_err_per_instance(outs[1], outs[0]))
@@ -306,8 +312,10 @@ def main():
         from oracle import rgf_ref
         g = groups[0]
         sh = None if g["shift"] is None else g["shift"][:1]
-        p = core.RgfPlan(*g["dev"], shift=sh, dscale=g["dscale"], oscale=g["oscale"])
+        p = core.RgfPlan(*g["dev"], shift=sh, dscale=g["dscale"], oscale=g["oscale"],
+                         inverse=args.inverse)
         p.run()
+        p.check()
         got = tuple(x[0].cpu().numpy() for x in p.outputs())
         if g["shift"] is not None:
             d, u, l = configs.shifted(g["H"], g["shift"][0])
@@ -335,7 +343,8 @@ def main():
         for g in groups:
             nE = len(g["shift"]) if g["shift"] is not None else 1
             ps = [core.RgfPlan(*g["dev"], shift=None if g["shift"] is None else g["shift"][k:k + 1],
-                               dscale=g["dscale"], oscale=g["oscale"]) for k in range(min(nsamp, nE))]
+                               dscale=g["dscale"], oscale=g["oscale"], inverse=args.inverse)
+                  for k in range(min(nsamp, nE))]
             ps[0].run()
             torch.cuda.synchronize()
 
@@ -577,7 +586,8 @@ def main():
                   "int8x": "int8-digit-gemm/complex128"}[head],
         "data": "synthetic",
         "config": dict(wl, launch="eager" if args.eager else
-                       "hip graph replay of one captured step (groups on their own streams)"),
+                       "hip graph replay of one captured step (groups on their own streams)",
+                       inverse={str(g["b"]): g.get("inverse") for g in groups}),
         "tflops": round(alg_flops / (ms / 1e3) / 1e12, 3),
         "headline_mode": head,
         "modes": {
@@ -989,18 +999,21 @@ def bench_convert(args):
 
 def bench_variants(args):
     """One large FP64 instance (128 blocks of 512, the diagonally dominant
-    generator) through every selected-inversion method: RGF, BCR-serial,
-    RGF-2sided, PSR and PDIV with their partitions batched on the GPU.  Per
-    method: device ms per solve, executed TFLOP/s (launch profiler flops over
-    the wall time of the solve) and relative error against RGF."""
+    generator) through every selected-inversion method: RGF (block inverses by
+    the Newton-Schulz iteration, which "auto" picks for this dominant instance,
+    and by the pivoted elimination), BCR-serial, RGF-2sided, PSR and PDIV with
+    their partitions batched on the GPU.  Per method: device ms per solve,
+    executed TFLOP/s (launch profiler flops over the wall time of the solve)
+    and relative error against RGF with the pivoted elimination."""
     import torch
     from sinv_amd import core, _native, autotune
     from sinv_amd.utils import configs
     N, b = 128, 512
     d, u, l = (core.to_dev(x) for x in configs.dominant_blocks(N, b, seed=2))
-    ref = core.rgf_batched(d, u, l)
+    ref = core.rgf_batched(d, u, l, inverse="pivoted")
     methods = [
         ("rgf", lambda: core.rgf_batched(d, u, l, check=False)),
+        ("rgf_pivoted", lambda: core.rgf_batched(d, u, l, check=False, inverse="pivoted")),
         ("bcr_serial", lambda: core.bcr_batched(d, u, l, check=False)),
         ("rgf2sided", lambda: core.psr_batched(d, u, l, 2, first_rows=N // 2, check=False)),
         ("psr_p4", lambda: core.psr_batched(d, u, l, 4, check=False)),

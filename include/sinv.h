@@ -332,6 +332,39 @@ int sinv_gj_force_variant(int variant);
  * through info, so the kernel always exits. */
 int sinv_leaf_spin_limit(int polls);
 
+/* Inverse method of the calling thread (no counterpart in the reference:
+ * numpy.linalg.inv always eliminates).  SINV_INV_PIVOTED (default): the
+ * elimination above.  SINV_INV_NEWTON: launches of blocks above 64 with at
+ * most 4 instances (the latency-bound case, where the elimination's
+ * column-serial pivot chain occupies one CU) run the Newton-Schulz
+ * iteration X <- X (2I - M X) from X = 2 D^-1 - D^-1 M D^-1 (D = diag(M))
+ * on the FP64 GEMM instead.  An instance is accepted once
+ * max|I - M X_k| <= 1e-9, which makes the next iterate's residual at most
+ * (n 1e-9)^2 in norm; one whose residual stays above that after max_iter
+ * products (1 .. 16; 0 keeps the current limit, initially 6) is reported
+ * through info with SINV_INFO_NOT_CONVERGED set in the key, and its output
+ * is undefined: repeat the call with SINV_INV_PIVOTED.  The iteration
+ * converges whenever every inverted block is strictly column diagonally
+ * dominant (sum_{i != j} |m_ij| < |m_jj|, which every block RGF, BCR, PSR
+ * and PDIV invert inherits from a matrix with that property; see
+ * sinv_column_dominance), at a rate set by the dominance ratio.  Returns
+ * the previous method, or -1 for an unknown method. */
+#define SINV_INV_PIVOTED 0
+#define SINV_INV_NEWTON 1
+#define SINV_INFO_NOT_CONVERGED (1ULL << 62)
+int sinv_inverse_method(int method, int max_iter);
+
+/* Column dominance ratio of block-tridiagonal instances described as for
+ * sinv_rgf_batched (diag, upper, lower, strides, bsizes, batch, dscale,
+ * oscale, shift; the output blocks are not used): rho[z] =
+ * max_j sum_{i != j} |a_ij| / |a_jj| over the whole matrix of instance z
+ * (device array of batch doubles; +inf where a diagonal entry is zero).
+ * rho < 1 means strictly column diagonally dominant, so every Schur
+ * complement the selected-inversion methods invert is too, with a ratio no
+ * larger (no counterpart in the reference; it decides when
+ * SINV_INV_NEWTON applies). */
+int sinv_column_dominance(const sinv_rgf_desc* d, double* rho, void* stream);
+
 /* ------------------------------------------------------------------------
  * Batched complex GEMM (kernel K2):
  *   C[z] = alpha * op(A[z]) op(B[z]) + beta * C0[z] (+ shift[z] * I)

@@ -15,6 +15,10 @@ SINV_E_SINGULAR = 1
 SINV_E_INVAL = 2
 SINV_E_HIP = 3
 SINV_INFO_OK = (1 << 64) - 1
+SINV_INFO_NOT_CONVERGED = 1 << 62
+
+# block inverse methods (include/sinv.h SINV_INV_*)
+INV_METHODS = {"pivoted": 0, "newton": 1}
 
 PREC = {"fp64": 0, "mixed": 1, "int8x": 2, "mixed_refined": 3}
 OPS = {"N": 0, "T": 1, "C": 2}
@@ -179,6 +183,10 @@ def lib():
     L.sinv_gemm_tune_clear.argtypes = []
     L.sinv_gemm_select.restype = _i
     L.sinv_gemm_select.argtypes = [_i, _i, _i, _ll, _i, _i, _i, _i]
+    L.sinv_inverse_method.restype = _i
+    L.sinv_inverse_method.argtypes = [_i, _i]
+    L.sinv_column_dominance.restype = _i
+    L.sinv_column_dominance.argtypes = [ctypes.POINTER(RgfDesc), _vp, _vp]
     L.sinv_prof_records.restype = _i
     L.sinv_prof_records.argtypes = [_i, ctypes.POINTER(_i), ctypes.POINTER(_ll),
                                     ctypes.POINTER(ctypes.c_float)]
@@ -218,19 +226,54 @@ def check(rc, what):
     raise RuntimeError(f"{what}: HIP error: {msg}")
 
 
+class NotConverged(RuntimeError):
+    """A Newton-Schulz block inverse did not reach its acceptance bound (the
+    call's outputs are undefined; repeat it with the pivoted elimination)."""
+
+
 def check_info(info_tensor, what="inverse"):
-    """Raise numpy's LinAlgError (as numpy.linalg.inv does) if a pivot was zero.
+    """Raise numpy's LinAlgError (as numpy.linalg.inv does) if a pivot was zero,
+    or NotConverged if a Newton-Schulz inverse missed its bound.
 
     Synchronises the stream of ``info_tensor``."""
     v = int(info_tensor.cpu().numpy().view(np.uint64)[0])
     if v != SINV_INFO_OK:
-        block, inst = v >> 32, v & 0xFFFFFFFF
-        err = np.linalg.LinAlgError("Singular matrix")
-        err.block, err.instance = block, inst
+        if v & SINV_INFO_NOT_CONVERGED:
+            v &= ~SINV_INFO_NOT_CONVERGED
+            err = NotConverged(f"{what}: Newton-Schulz inverse of block {v >> 32}, instance "
+                               f"{v & 0xFFFFFFFF} did not converge")
+        else:
+            err = np.linalg.LinAlgError("Singular matrix")
+        err.block, err.instance = v >> 32, v & 0xFFFFFFFF
         raise err
 
 
-PROF_KINDS = ("gemm_f64", "gemm_f32", "inv_leaf", "other", "gemm_i8x", "inv_panel")
+class inverse_method:
+    """Context manager selecting the block-inverse method of library calls made
+    on this thread: "pivoted" (Gauss-Jordan with partial pivoting, numpy's
+    semantics) or "newton" (Newton-Schulz iteration for launches of at most 4
+    blocks above 64, for column diagonally dominant blocks; see
+    include/sinv.h).  Workspace queries made inside the block size for it."""
+
+    def __init__(self, method, max_iter=0):
+        if method not in INV_METHODS:
+            raise ValueError(f"inverse method must be one of {sorted(INV_METHODS)}")
+        self.method, self.max_iter = INV_METHODS[method], int(max_iter)
+
+    def __enter__(self):
+        prev = lib().sinv_inverse_method(self.method, self.max_iter)
+        if prev < 0:
+            raise ValueError(lib().sinv_last_error().decode(errors="replace"))
+        self.prev = prev
+        return self
+
+    def __exit__(self, *exc):
+        # a limit given here is put back to the library's default (6)
+        lib().sinv_inverse_method(self.prev, 6 if self.max_iter else 0)
+        return False
+
+
+PROF_KINDS = ("gemm_f64", "gemm_f32", "inv_leaf", "other", "gemm_i8x", "inv_panel", "inv_newton")
 
 # leaf inverse kernels (include/sinv.h SINV_LEAF_*)
 LEAF_VARIANTS = {"auto": 1, "lds": 0, "rows8": 2, "panel8": 3, "panel16": 4, "panel16f": 5,

@@ -12,6 +12,7 @@ instance.
 """
 
 import ctypes
+import math
 
 import numpy as np
 import torch
@@ -104,8 +105,98 @@ def _resolve_batch(*cands):
     return bs.pop() if bs else 1
 
 
+# ---------------------------------------------------------------- inverse method
+INVERSE_CHOICES = ("auto", "pivoted", "newton")
+NEWTON_MAX_BATCH = 4      # include/sinv.h: launches of at most 4 blocks
+NEWTON_ACCEPT = 1e-9      # acceptance bound on max |I - M X_k|
+NEWTON_AUTO_MAX_ITER = 8  # "auto" takes the iteration only when this many steps suffice
+
+
+def newton_iterations(rho):
+    """Steps of the Newton-Schulz block inverse that are enough for blocks whose
+    column dominance ratio is at most ``rho``: from the Jacobi start,
+    ||I - M X_k||_1 <= rho^(2^k), so step k passes the acceptance bound once
+    rho^(2^k) <= 1e-9 (max |.| <= ||.||_1).  None if rho >= 1."""
+    rho = float(rho)
+    if not rho < 1.0:
+        return None
+    if rho < 1e-300:
+        return 1
+    return max(1, math.ceil(math.log2(math.log(NEWTON_ACCEPT) / math.log(rho))))
+
+
+def column_dominance(diag, upper, lower, *, shift=None, dscale=1.0, oscale=1.0, bsizes=None):
+    """Column dominance ratio max_j sum_{i != j} |a_ij| / |a_jj| of each
+    block-tridiagonal instance (the RGF convention for diag / upper / lower,
+    shift, dscale, oscale), as a float64 tensor on the device.  Below 1 the
+    matrix is strictly column diagonally dominant, and so is every block the
+    selected-inversion methods invert from it (ratio no larger)."""
+    d, Bd, sd = _batched(to_dev(diag), 3)
+    N = d.shape[-3]
+    if N > 1:
+        u, Bu, su = _batched(to_dev(upper), 3)
+        l, Bl, sl = _batched(to_dev(lower), 3)
+    else:
+        u = l = None
+        Bu = Bl = None
+        su = sl = 0
+    sh = to_dev(shift).reshape(-1) if shift is not None else None
+    B = _resolve_batch(Bd, Bu, Bl, None if sh is None else sh.numel())
+    return _dominance(d, sd, u, su, l, sl, sh, B, N, d.shape[-1], dscale, oscale)
+
+
+def _dominance(d, sd, u, su, l, sl, sh, B, N, b, dscale, oscale):
+    rho = torch.empty(B, dtype=torch.float64, device=device())
+    desc = nat.RgfDesc(nblocks=N, blocksize=b, bsizes=None, batch=B, precision=0,
+                       diag=_p(d), diag_stride=sd, upper=_p(u), upper_stride=su,
+                       lower=_p(l), lower_stride=sl, dscale=nat.C128.of(dscale),
+                       oscale=nat.C128.of(oscale), shift=_p(sh))
+    nat.check(nat.lib().sinv_column_dominance(ctypes.byref(desc), _p(rho), _stream()),
+              "column_dominance")
+    return rho
+
+
+def _choose_inverse(inverse, B, b, rho_fn):
+    """(method, iteration limit) for a call of B instances of b-blocks."""
+    if inverse not in INVERSE_CHOICES:
+        raise ValueError(f"inverse must be one of {INVERSE_CHOICES}")
+    if inverse == "pivoted":
+        return "pivoted", 0
+    if inverse == "newton":
+        return "newton", 0
+    # the dominance test synchronises, which a stream under graph capture
+    # cannot: there "auto" takes the elimination (pass "newton" to capture
+    # the iteration)
+    if B > NEWTON_MAX_BATCH or b <= 64 or torch.cuda.is_current_stream_capturing():
+        return "pivoted", 0
+    k = newton_iterations(rho_fn().max().item())
+    if k is None or k > NEWTON_AUTO_MAX_ITER:
+        return "pivoted", 0
+    return "newton", k
+
+
+def _run_with_inverse(inverse, B, b, rho_fn, launch, info, check, what):
+    """Run ``launch()`` (workspace query and library call) under the inverse
+    method ``inverse`` resolves to; with "auto", a Newton-Schulz block inverse
+    that misses its bound makes the call run again with the elimination."""
+    method, iters = _choose_inverse(inverse, B, b, rho_fn)
+    with nat.inverse_method(method, iters):
+        launch()
+    if check or (method == "newton" and inverse == "auto"):
+        try:
+            nat.check_info(info, what)
+        except nat.NotConverged:
+            if inverse != "auto":
+                raise
+            with nat.inverse_method("pivoted"):
+                launch()
+            if check:
+                nat.check_info(info, what)
+    return method
+
+
 def rgf_batched(diag, upper, lower, *, sym_mat=False, save_off_diag=True, precision="fp64",
-                shift=None, dscale=1.0, oscale=1.0, check=True):
+                shift=None, dscale=1.0, oscale=1.0, check=True, inverse="auto"):
     """RGF selected inversion of a batch of block-tridiagonal instances.
 
     Instance z is A_ii = dscale*diag[z][i] + shift[z]*I, A_{i,i+1} = oscale*upper[z][i],
@@ -127,6 +218,20 @@ def rgf_batched(diag, upper, lower, *, sym_mat=False, save_off_diag=True, precis
       "mixed_refined"  complex64 forward products, one FP64 refinement step of
                        each g_i, FP64 backward sweep: up to ~1e-7 on the same
                        instances.
+
+    inverse: how the blocks M_i are inverted (rgf.py:44,48 call numpy.linalg.inv):
+      "pivoted"  Gauss-Jordan elimination with partial pivoting over all rows,
+                 numpy's semantics, for every input;
+      "newton"   Newton-Schulz iteration on the FP64 GEMM for calls of at most
+                 4 instances (blocks above 64), which converges for column
+                 diagonally dominant blocks; raises NotConverged otherwise;
+      "auto"     (default) "newton" when the call is latency-bound (at most 4
+                 instances of blocks above 64) and the matrix is strictly
+                 column diagonally dominant (``column_dominance`` below 1, one
+                 device reduction and synchronisation), with the iteration
+                 limit that ratio guarantees; "pivoted" otherwise, when an
+                 iteration misses its bound, and on a stream under graph
+                 capture (which cannot synchronise).
     """
     if precision not in nat.PREC:
         raise ValueError(f"precision must be one of {sorted(nat.PREC)}")
@@ -160,20 +265,28 @@ def rgf_batched(diag, upper, lower, *, sym_mat=False, save_off_diag=True, precis
         dscale=nat.C128.of(dscale), oscale=nat.C128.of(oscale), shift=_p(sh),
         g_diag=_p(Gd), g_upper=_p(Gu), g_lower=_p(Gl), info=_info_ptr(info))
     L = nat.lib()
-    ws = _workspace(L.sinv_rgf_workspace_bytes(ctypes.byref(desc)))
-    nat.check(L.sinv_rgf_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()), "rgf")
-    if check:
-        nat.check_info(info)
+
+    def launch():
+        ws = _workspace(L.sinv_rgf_workspace_bytes(ctypes.byref(desc)))
+        nat.check(L.sinv_rgf_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()), "rgf")
+
+    _run_with_inverse(inverse, B, b,
+                      lambda: _dominance(d, sd, u, su, l, sl, sh, B, N, b, dscale, oscale),
+                      launch, info, check, "rgf")
     return Gd, Gu, Gl
 
 
 class RgfPlan:
     """Pre-allocated batched RGF: inputs, outputs, workspace and descriptor are
     bound once; ``run()`` only enqueues kernels on the current stream (nothing
-    is allocated or synchronised).  Same semantics as ``rgf_batched``."""
+    is allocated or synchronised).  Same semantics as ``rgf_batched``; the
+    inverse method (default "pivoted") is fixed at construction, "auto"
+    deciding on the inputs bound then, and ``check()`` raises NotConverged if
+    a Newton-Schulz inverse missed its bound."""
 
     def __init__(self, diag, upper, lower, *, batch=None, sym_mat=False, save_off_diag=True,
-                 precision="fp64", shift=None, dscale=1.0, oscale=1.0, out=None):
+                 precision="fp64", shift=None, dscale=1.0, oscale=1.0, out=None,
+                 inverse="pivoted"):
         if precision not in nat.PREC:
             raise ValueError(f"precision must be one of {sorted(nat.PREC)}")
         _check_tridiag(diag, upper, lower)
@@ -203,12 +316,19 @@ class RgfPlan:
             oscale=nat.C128.of(oscale), shift=_p(self.shift), g_diag=_p(self.Gd),
             g_upper=_p(self.Gu), g_lower=_p(self.Gl), info=_info_ptr(self.info))
         self.lib = nat.lib()
-        self.ws = _workspace(self.lib.sinv_rgf_workspace_bytes(ctypes.byref(self.desc)))
+        # the method is fixed here ("auto" decides on the inputs bound now; a
+        # plan whose inputs change later may choose "pivoted" explicitly)
+        self.inverse = _choose_inverse(
+            inverse, B, b, lambda: _dominance(self.d, sd, self.u, su, self.l, sl, self.shift, B,
+                                              N, b, dscale, oscale))
+        with nat.inverse_method(*self.inverse):
+            self.ws = _workspace(self.lib.sinv_rgf_workspace_bytes(ctypes.byref(self.desc)))
         self.N, self.b, self.batch = N, b, B
 
     def run(self):
-        nat.check(self.lib.sinv_rgf_batched(ctypes.byref(self.desc), _p(self.ws),
-                                             self.ws.numel(), _stream()), "rgf")
+        with nat.inverse_method(*self.inverse):
+            nat.check(self.lib.sinv_rgf_batched(ctypes.byref(self.desc), _p(self.ws),
+                                                 self.ws.numel(), _stream()), "rgf")
 
     def check(self):
         nat.check_info(self.info)
@@ -275,7 +395,7 @@ class PlanSet:
 
 
 def bcr_batched(diag, upper, lower, *, shift=None, dscale=1.0, oscale=1.0, return_fill=False,
-                check=True, npart=1):
+                check=True, npart=1, inverse="auto"):
     """BCR-serial selected inversion of a batch of instances (bcr_serial.py:19-466).
 
     Same instance convention as ``rgf_batched`` (uniform block size).  Returns
@@ -283,7 +403,9 @@ def bcr_batched(diag, upper, lower, *, shift=None, dscale=1.0, oscale=1.0, retur
     of levels >= 1 as (B, F, b, b) in the order documented in include/sinv.h.
     ``npart > 1`` issues every level operation per partition of the padded
     block rows, as BCR-parallel's ranks split them (sinv_bcr_parallel_batched;
-    same bits).
+    same bits).  ``inverse`` as for ``rgf_batched``; with "newton" the
+    iteration serves the launches of at most 4 blocks (the upper levels and
+    the root), the elimination the larger ones.
     """
     _check_tridiag(diag, upper, lower)
     d, Bd, sd = _batched(to_dev(diag), 3)
@@ -310,14 +432,18 @@ def bcr_batched(diag, upper, lower, *, shift=None, dscale=1.0, oscale=1.0, retur
         lower=_p(l), lower_stride=sl, dscale=nat.C128.of(dscale), oscale=nat.C128.of(oscale),
         shift=_p(sh), g_diag=_p(Gd), g_upper=_p(Gu), g_lower=_p(Gl), g_fill=_p(Gf),
         info=_info_ptr(info))
-    ws = _workspace(L.sinv_bcr_workspace_bytes(ctypes.byref(desc)))
-    if npart == 1:
-        nat.check(L.sinv_bcr_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()), "bcr")
-    else:
-        nat.check(L.sinv_bcr_parallel_batched(ctypes.byref(desc), int(npart), _p(ws), ws.numel(),
-                                              _stream()), "bcr_parallel")
-    if check:
-        nat.check_info(info)
+    def launch():
+        ws = _workspace(L.sinv_bcr_workspace_bytes(ctypes.byref(desc)))
+        if npart == 1:
+            nat.check(L.sinv_bcr_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()),
+                      "bcr")
+        else:
+            nat.check(L.sinv_bcr_parallel_batched(ctypes.byref(desc), int(npart), _p(ws),
+                                                  ws.numel(), _stream()), "bcr_parallel")
+
+    _run_with_inverse(inverse, B, b,
+                      lambda: _dominance(d, sd, u, su, l, sl, sh, B, N, b, dscale, oscale),
+                      launch, info, check, "bcr")
     if return_fill:
         if Gf is None:
             Gf = torch.empty((B, 0, b, b), dtype=_c128, device=dev)
@@ -524,12 +650,13 @@ def rgf_adjoint_ragged(diag, upper, lower, gbar_diag, gbar_upper, gbar_lower, *,
 
 
 def psr_batched(diag, upper, lower, npart, *, shift=None, dscale=1.0, oscale=1.0, check=True,
-                first_rows=0):
+                first_rows=0, inverse="auto"):
     """PSR selected inversion (psr_seqsolve.py:20-72) with ``npart`` partitions
     on one GPU: partition-local Schur reduction and production batched over
     partitions, the 2(npart-1)-block reduced system solved by RGF.  Needs
-    nblocks >= 3 * npart (psr_utils.check_input).  Same instance convention and
-    outputs as ``rgf_batched``."""
+    nblocks >= 3 * npart (psr_utils.check_input).  Same instance convention,
+    outputs and ``inverse`` choices as ``rgf_batched`` (the iteration serves
+    launches of at most 4 blocks)."""
     N, b = _check_tridiag(diag, upper, lower)
     if first_rows:
         if npart != 2 or not 1 <= first_rows < N:
@@ -554,10 +681,14 @@ def psr_batched(diag, upper, lower, npart, *, shift=None, dscale=1.0, oscale=1.0
         oscale=nat.C128.of(oscale), shift=_p(sh), g_diag=_p(Gd), g_upper=_p(Gu), g_lower=_p(Gl),
         info=_info_ptr(info))
     L = nat.lib()
-    ws = _workspace(L.sinv_psr_workspace_bytes(ctypes.byref(desc)))
-    nat.check(L.sinv_psr_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()), "psr")
-    if check:
-        nat.check_info(info)
+
+    def launch():
+        ws = _workspace(L.sinv_psr_workspace_bytes(ctypes.byref(desc)))
+        nat.check(L.sinv_psr_batched(ctypes.byref(desc), _p(ws), ws.numel(), _stream()), "psr")
+
+    _run_with_inverse(inverse, B, b,
+                      lambda: _dominance(d, sd, u, su, l, sl, sh, B, N, b, dscale, oscale),
+                      launch, info, check, "psr")
     return Gd, Gu, Gl
 
 
@@ -687,16 +818,24 @@ def scatter_blocks(A, blocks, positions):
     return A
 
 
-def zinv_batched(A, check=True):
-    """In-place-style batched inverse of (..., n, n) complex128 blocks; returns a new tensor."""
+def zinv_batched(A, check=True, inverse="pivoted", max_iter=0):
+    """In-place-style batched inverse of (..., n, n) complex128 blocks; returns a new tensor.
+
+    inverse="pivoted": Gauss-Jordan elimination with partial pivoting (numpy's
+    semantics); "newton": the Newton-Schulz iteration for batches of at most 4
+    blocks above 64 (``max_iter`` steps at most, 0 = the library's limit),
+    raising NotConverged when a block misses the acceptance bound."""
+    if inverse not in ("pivoted", "newton"):
+        raise ValueError('inverse must be "pivoted" or "newton"')
     X = to_dev(A).clone()
     n = X.shape[-1]
     batch = int(np.prod(X.shape[:-2])) if X.dim() > 2 else 1
     info = torch.empty(1, dtype=torch.int64, device=X.device)
     L = nat.lib()
-    ws = _workspace(L.sinv_zinv_workspace_bytes(n, batch))
-    nat.check(L.sinv_zinv_batched(_p(X), n, n, n * n, batch, _p(ws), ws.numel(), _info_ptr(info),
-                                  _stream()), "zinv")
+    with nat.inverse_method(inverse, max_iter):
+        ws = _workspace(L.sinv_zinv_workspace_bytes(n, batch))
+        nat.check(L.sinv_zinv_batched(_p(X), n, n, n * n, batch, _p(ws), ws.numel(),
+                                      _info_ptr(info), _stream()), "zinv")
     if check:
         nat.check_info(info)
     return X

@@ -82,9 +82,10 @@ Plan make_plan(int N, int b, int batch) {
     }
   }
   P.per_inst = off;
-  P.inv_elems = inverse_workspace_elems(b, batch * (maxelim > 0 ? maxelim : 1));
+  const bool newton = inverse_method() == SINV_INV_NEWTON;
+  P.inv_elems = inverse_workspace_elems(b, batch * (maxelim > 0 ? maxelim : 1), newton);
   // root inverse runs with batch instances only
-  const size_t root_inv = inverse_workspace_elems(b, batch);
+  const size_t root_inv = inverse_workspace_elems(b, batch, newton);
   if (root_inv > P.inv_elems) P.inv_elems = root_inv;
   P.bytes = ((size_t)P.per_inst * (size_t)b2 * (size_t)batch + P.inv_elems) * sizeof(cplx) + 512;
   return P;
@@ -155,6 +156,7 @@ int bcr_run(const sinv_bcr_desc* d, int npart, void* work, size_t work_bytes, vo
   cplx* W = base;                                // per-instance blocks, instance stride per_inst
   cplx* winv = base + P.per_inst * b2 * nb;      // inverse recursion scratch
   const long long WI = P.per_inst * b2;          // instance stride of W
+  const bool newton = inverse_method() == SINV_INV_NEWTON;  // as in make_plan
 
   const cplx one{1.0, 0.0}, mone{-1.0, 0.0}, zero{0.0, 0.0};
   const cplx osc{d->oscale.re, d->oscale.im};
@@ -258,7 +260,7 @@ int bcr_run(const sinv_bcr_desc* d, int npart, void* work, size_t work_bytes, vo
                                       2LL * sl * b2, WI, W + (Lv.ainv + q0) * b2, b, b2, WI,
                                       nullptr, q1 - q0, nb, s));
       return launch_inverse(W + (Lv.ainv + q0) * b2, b, b, b2, WI, q1 - q0, nb, winv, d->info,
-                            (int)elim(q0), s);
+                            (int)elim(q0), s, newton);
     }));
     const int ce = (n - 1 + 1) / 2;  // even couplings c = 2q < n-1
     const int co = (n - 1) / 2;      // odd couplings c = 2q+1 < n-1
@@ -290,7 +292,7 @@ int bcr_run(const sinv_bcr_desc* d, int npart, void* work, size_t work_bytes, vo
   SINV_HIP_TRY(launch_axpby_shift(b, b, one, Dw.p + (long long)P.root * b2, b, 0, WI,
                                   Gd + (long long)P.root * b2, b, 0, GdI, nullptr, 1, nb, s));
   SINV_HIP_TRY(launch_inverse(Gd + (long long)P.root * b2, b, b, 0, GdI, 1, nb, winv, d->info,
-                              P.root, s));
+                              P.root, s, newton));
 
   // ---------------------------------------------------------------- production
   const Arr GD{Gd, GdI};

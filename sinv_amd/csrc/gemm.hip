@@ -164,6 +164,8 @@ void gemm_kernel(GemmArgs g, int tiles_m, int tiles_n) {
   auto& Bs2 = *reinterpret_cast<CR<R>(*)[2][KC][BNt + PAD]>(lds_raw + kAElems);
 
   const TileMap tm = map_tile<BMt, BNt>(tiles_m, tiles_n, g.inner);
+  // converged instances of an iteration leave at once (uniform per workgroup)
+  if (g.skip != nullptr && g.skip[tm.z_out * g.inner + tm.z_in] < g.skip_iter) return;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -518,14 +520,20 @@ using Cfg32x64 = Cfg<2, 2, 1, 2, 3>;  // 6: 32 x 64, 4 waves of 16 x 32
 using Cfg32k4 = Cfg<1, 1, 2, 2, 1, 4>;  // 7: 32 x 32, 4 k-slices of 32 x 32 waves
 using Cfg16k4 = Cfg<1, 1, 1, 1, 2, 4>;  // 8: 16 x 16, 4 k-slices
 using Cfg16k8 = Cfg<1, 1, 1, 1, 1, 8>;  // 9: 16 x 16, 8 k-slices
-constexpr int kNumCfg = 10;
+// the 32 x 32 tile of config 1 (4 waves of 16 x 16) with its k split 4 or 2
+// ways: 16 (8) waves per workgroup, so one workgroup per CU still puts 4 (2)
+// waves on each SIMD to cover the chunk loads of an unbatched 512^3 product
+using Cfg32w4k4 = Cfg<2, 2, 1, 1, 4, 4>;  // 10: 32 x 32, 4 k-slices of 4 waves
+using Cfg32w4k2 = Cfg<2, 2, 1, 1, 4, 2>;  // 11: 32 x 32, 2 k-slices of 4 waves
+constexpr int kNumCfg = 12;
 constexpr int kCfgBM[kNumCfg] = {Cfg64::BM,     Cfg32w4::BM,  Cfg32w1::BM,  Cfg128x64::BM,
                                  Cfg64x128::BM, Cfg64x32::BM, Cfg32x64::BM, Cfg32k4::BM,
-                                 Cfg16k4::BM,   Cfg16k8::BM};
+                                 Cfg16k4::BM,   Cfg16k8::BM,  Cfg32w4k4::BM, Cfg32w4k2::BM};
 constexpr int kCfgBN[kNumCfg] = {Cfg64::BN,     Cfg32w4::BN,  Cfg32w1::BN,  Cfg128x64::BN,
                                  Cfg64x128::BN, Cfg64x32::BN, Cfg32x64::BN, Cfg32k4::BN,
-                                 Cfg16k4::BN,   Cfg16k8::BN};
-constexpr int kCfgWK[kNumCfg] = {1, 1, 1, 1, 1, 1, 1, Cfg32k4::WK, Cfg16k4::WK, Cfg16k8::WK};
+                                 Cfg16k4::BN,   Cfg16k8::BN,  Cfg32w4k4::BN, Cfg32w4k2::BN};
+constexpr int kCfgWK[kNumCfg] = {1, 1, 1, 1, 1, 1, 1, Cfg32k4::WK, Cfg16k4::WK, Cfg16k8::WK,
+                                 Cfg32w4k4::WK, Cfg32w4k2::WK};
 
 template <typename R, typename C>
 void launch_cfg(const GemmArgs& g, long long batch, hipStream_t s) {
@@ -547,6 +555,8 @@ void launch_by_id(int id, const GemmArgs& g, long long batch, hipStream_t s) {
     case 7: launch_cfg<R, Cfg32k4>(g, batch, s); break;
     case 8: launch_cfg<R, Cfg16k4>(g, batch, s); break;
     case 9: launch_cfg<R, Cfg16k8>(g, batch, s); break;
+    case 10: launch_cfg<R, Cfg32w4k4>(g, batch, s); break;
+    case 11: launch_cfg<R, Cfg32w4k2>(g, batch, s); break;
     default: launch_cfg<R, Cfg64>(g, batch, s); break;
   }
 }
@@ -576,15 +586,20 @@ int tune_find(const GemmArgs& g, long long batch, int prec) {
 
 // Untuned shapes: the 64 x 64 tile, unless its grid leaves the chip under half
 // full (under 2 workgroups per CU), where 32 x 32 tiles of 4 waves give 4x the
-// workgroups; if even those leave CUs idle (an unbatched product of 256^3 or
-// less), 16 x 16 tiles with k split four ways inside the workgroup (measured
-// 64^3 4.2 us, 128^3 5.2 us, 256^3 7.5 us per launch against 5.8, 8.8 and
-// 15.0 us with the 32 x 32 tile).
+// workgroups.  Grids of 129 .. 640 such tiles (about one per CU) take the same
+// tile with its k split 2 ways (config 11), or 4 ways up to 256 tiles with
+// K >= 384 (config 10): two or four waves per SIMD then cover the chunk loads
+// (graph-timed per launch: 512^3 21.8 / 27.0 us, 416^3 19.4 / 23.1 us,
+// 384^3 x 2 29.1 / 31.8 us, 256^3 x 4 13.8 / 16.1 us against the plain 32 x 32
+// tile).  Up to 128 such tiles (unbatched products of 256^3 or less), 16 x 16
+// tiles with k split four ways (64^3 4.3 us, 128^3 5.4 us, 256^3 7.4 us
+// against 5.8, 8.9 and 14.8 us with the 32 x 32 tile).
 int default_cfg(const GemmArgs& g, long long batch) {
   const long long t64 = (long long)((g.M + 63) / 64) * ((g.N + 63) / 64) * batch;
-  if (t64 >= 512) return 0;
   const long long t32 = (long long)((g.M + 31) / 32) * ((g.N + 31) / 32) * batch;
-  return (t32 < 256 && g.K >= 48) ? 8 : 1;
+  if (t32 <= 128) return g.K >= 48 ? 8 : 1;
+  if (t32 <= 640 && g.K >= 64) return (t32 <= 256 && g.K >= 384) ? 10 : 11;
+  return t64 >= 512 ? 0 : 1;
 }
 
 int g_force_cfg = -1;  // >= 0: every launch uses this config (tuning runs)
@@ -691,8 +706,11 @@ hipError_t launch_gemm(const GemmArgs& g, Prec p, hipStream_t s) {
   prof_shape(pid, g.M, g.N, g.K, batch, g.opA | (g.opB << 2) | (g.C0 != nullptr ? 16 : 0));
   const double elems = (double)g.M * g.K + (double)g.K * g.N +
                        (double)g.M * g.N * (g.C0 != nullptr ? 2.0 : 1.0);
-  prof_close(pid, s, p == Prec::F64 ? PROF_GEMM_F64 : PROF_GEMM_F32,
-             8.0 * g.M * (double)g.N * g.K * nb, sizeof(cplx) * elems * nb);
+  if (g.prof_as > 0)
+    prof_close(pid, s, g.prof_as - 1, 0.0, sizeof(cplx) * elems * nb);
+  else
+    prof_close(pid, s, p == Prec::F64 ? PROF_GEMM_F64 : PROF_GEMM_F32,
+               8.0 * g.M * (double)g.N * g.K * nb, sizeof(cplx) * elems * nb);
   return hipGetLastError();
 }
 

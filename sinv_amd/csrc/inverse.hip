@@ -1785,11 +1785,20 @@ hipError_t gj_range(cplx* A, int n, int lda, long long s_in, long long s_out, in
 
 }  // namespace
 
-size_t inverse_workspace_elems(int n, int batch) {
+// elimination: the eliminated-matrix copy (also the sweeps' B operand), then
+// positions and pivots
+static size_t gj_workspace_elems(int n, long long nb) {
+  return (size_t)n * n * (size_t)nb + ((size_t)n * (size_t)nb + 1) / 2 + 1;
+}
+
+size_t inverse_workspace_elems(int n, int batch, bool newton) {
   if (n <= LEAF) return 0;
-  const size_t nb = (size_t)batch;
-  // eliminated-matrix copy (also the sweeps' B operand), then positions and pivots
-  return (size_t)n * n * nb + ((size_t)n * nb + 1) / 2 + 1;
+  // the Newton-Schulz buffers follow the elimination's
+  return gj_workspace_elems(n, batch) + (newton ? newton_workspace_elems(n) : 0);
+}
+
+static bool use_newton(bool newton, int n, long long nb) {
+  return newton && n > LEAF && nb <= kNewtonMaxBatch;
 }
 
 namespace {
@@ -1847,11 +1856,14 @@ hipError_t gj_eliminate(cplx* A, int n, int lda, long long s_in, long long s_out
 
 hipError_t launch_inverse(cplx* A, int n, int lda, long long s_in, long long s_out, int inner,
                           int outer, cplx* work, unsigned long long* info, int tag,
-                          hipStream_t s) {
+                          hipStream_t s, bool newton) {
   if (n <= 0 || inner <= 0 || outer <= 0) return hipSuccess;
   if (n <= LEAF) return launch_leaf(A, n, lda, s_in, s_out, inner, outer, info, tag, s);
   if (n > kMaxInverseN || work == nullptr) return hipErrorInvalidValue;
   const long long nb = (long long)inner * outer;
+  if (use_newton(newton, n, nb))
+    return launch_newton_inverse(A, n, lda, s_in, s_out, inner, outer, A, lda, s_in, s_out, true,
+                                 work, gj_workspace_elems(n, nb), info, tag, s);
   int *ipos, *ipiv;
   SINV_HIP_RET(gj_eliminate(A, n, lda, s_in, s_out, inner, outer, work, &ipos, &ipiv, info, tag,
                             s));
@@ -1869,7 +1881,8 @@ hipError_t launch_inverse(cplx* A, int n, int lda, long long s_in, long long s_o
 
 hipError_t launch_inverse_to(cplx* A, int n, int lda, long long s_in, long long s_out, int inner,
                              int outer, cplx* X, int ldx, long long sx_in, long long sx_out,
-                             cplx* work, unsigned long long* info, int tag, hipStream_t s) {
+                             cplx* work, unsigned long long* info, int tag, hipStream_t s,
+                             bool newton) {
   if (n <= 0 || inner <= 0 || outer <= 0) return hipSuccess;
   if (n <= LEAF) {
     SINV_HIP_RET(launch_axpby_shift(n, n, cplx{1.0, 0.0}, A, lda, s_in, s_out, X, ldx, sx_in,
@@ -1878,6 +1891,9 @@ hipError_t launch_inverse_to(cplx* A, int n, int lda, long long s_in, long long 
   }
   if (n > kMaxInverseN || work == nullptr) return hipErrorInvalidValue;
   const long long nb = (long long)inner * outer;
+  if (use_newton(newton, n, nb))
+    return launch_newton_inverse(A, n, lda, s_in, s_out, inner, outer, X, ldx, sx_in, sx_out,
+                                 false, work, gj_workspace_elems(n, nb), info, tag, s);
   int *ipos, *ipiv;
   SINV_HIP_RET(gj_eliminate(A, n, lda, s_in, s_out, inner, outer, work, &ipos, &ipiv, info, tag,
                             s));
@@ -1892,7 +1908,9 @@ hipError_t launch_inverse_to(cplx* A, int n, int lda, long long s_in, long long 
 
 extern "C" size_t sinv_zinv_workspace_bytes(int n, int batch) {
   if (n <= 0 || batch <= 0) return 0;
-  return sinv::inverse_workspace_elems(n, batch) * sizeof(sinv::cplx);
+  return sinv::inverse_workspace_elems(n, batch,
+                                       sinv::inverse_method() == SINV_INV_NEWTON) *
+         sizeof(sinv::cplx);
 }
 
 extern "C" int sinv_zinv_batched(void* A, int n, int lda, long long strideA, int batch, void* work,
@@ -1915,7 +1933,8 @@ extern "C" int sinv_zinv_batched(void* A, int n, int lda, long long strideA, int
   if (n == 0 || batch == 0) return SINV_OK;
   hipStream_t s = (hipStream_t)stream;
   if (info != nullptr) SINV_HIP_TRY(launch_fill_u64(info, 1, SINV_INFO_OK, s));
-  SINV_HIP_TRY(launch_inverse((cplx*)A, n, lda, strideA, 0, batch, 1, (cplx*)work, info, 0, s));
+  SINV_HIP_TRY(launch_inverse((cplx*)A, n, lda, strideA, 0, batch, 1, (cplx*)work, info, 0, s,
+                              inverse_method() == SINV_INV_NEWTON));
   return SINV_OK;
 }
 

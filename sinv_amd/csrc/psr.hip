@@ -84,8 +84,9 @@ PsrPlan psr_plan(int N, int b, int P, int batch, int first_rows) {
   Q.Rd = take(Q.R); Q.Ru = take(Q.R - 1); Q.Rl = take(Q.R - 1);
   Q.per_inst = o;
   const int maxc = P - 2 > 1 ? P - 2 : 1;
-  Q.inv_elems = inverse_workspace_elems(b, batch * maxc);
-  Q.inv_side_elems = inverse_workspace_elems(b, batch);
+  const bool newton = inverse_method() == SINV_INV_NEWTON;
+  Q.inv_elems = inverse_workspace_elems(b, batch * maxc, newton);
+  Q.inv_side_elems = inverse_workspace_elems(b, batch, newton);
   sinv_rgf_desc rd{};
   rd.nblocks = Q.R; rd.blocksize = b; rd.batch = batch; rd.precision = SINV_PREC_FP64;
   rd.save_off_diag = 1;
@@ -160,6 +161,7 @@ extern "C" int sinv_psr_batched(const sinv_psr_desc* d, void* work, size_t work_
   // workspace (wcur), as the reference's ranks run concurrently.
   hipStream_t cur = s;
   cplx* wcur = winv;
+  const bool newton = inverse_method() == SINV_INV_NEWTON;
   // C[k] (k < count; block index c0 + k*cs) = alpha A[a0 + k*as] B[b0 + k*bs] (+ C0[...])
   auto gemm = [&](int count, Arr A, long long a0, long long as, Arr B, long long b0,
                   long long bs, Arr C, long long c0, long long cs, cplx alpha, bool acc,
@@ -194,7 +196,7 @@ extern "C" int sinv_psr_batched(const sinv_psr_desc* d, void* work, size_t work_
   auto invert = [&](int count, long long r, long long st) -> hipError_t {
     SINV_HIP_RET(copy(count, Ad, r, st, invA, r, st, one));
     return launch_inverse(W + (Q.invA + r) * b2, b, b, st * b2, WI, count, nb, wcur, d->info,
-                          (int)r, cur);
+                          (int)r, cur, newton);
   };
 
   if (d->info != nullptr) SINV_HIP_TRY(launch_fill_u64(d->info, 1, SINV_INFO_OK, s));

@@ -48,8 +48,9 @@ WorkPlan plan_work(const Layout& L, int batch, int precision) {
   const size_t b2 = (size_t)L.bmax * L.bmax;
   w.tmp_elems = 3 * b2 * (size_t)batch;
   size_t inv = 0;
+  const bool newton = inverse_method() == SINV_INV_NEWTON;
   for (int i = 0; i < L.N; ++i) {
-    const size_t e = inverse_workspace_elems(L.b[i], batch);
+    const size_t e = inverse_workspace_elems(L.b[i], batch, newton);
     if (e > inv) inv = e;
   }
   w.inv_elems = inv;
@@ -182,12 +183,13 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
     dst = oop ? Q : Gd + L.doff[i];
     sdst = oop ? b2 : L.dsize;
   };
+  const bool newton = inverse_method() == SINV_INV_NEWTON;
   auto invert_m = [&](int i) -> hipError_t {
     const int bi = L.b[i];
     if (bi > kLeaf)
       return launch_inverse_to(Q, bi, bi, 0, b2, 1, nb, Gd + L.doff[i], bi, 0, L.dsize, winv,
-                               d->info, i, s);
-    return launch_inverse(Gd + L.doff[i], bi, bi, 0, L.dsize, 1, nb, winv, d->info, i, s);
+                               d->info, i, s, newton);
+    return launch_inverse(Gd + L.doff[i], bi, bi, 0, L.dsize, 1, nb, winv, d->info, i, s, newton);
   };
   {
     const int b0 = L.b[0];

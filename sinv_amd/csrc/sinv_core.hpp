@@ -36,6 +36,14 @@ struct GemmArgs {
   cplx alpha, beta;
   const cplx* shift;
   int inner, outer;  // batch = inner * outer
+  // optional per-instance early exit: instance z (= z_outer * inner + z_inner)
+  // is skipped when skip[z] < skip_iter (iterative inverse: instances that
+  // have converged stop); null = never skip.  FP64 / FP32 kernels only.
+  const int* skip;
+  int skip_iter;
+  // launch-profiler kind of this product + 1, recorded with no algorithmic
+  // flops (the iterative inverse's products); 0: a GEMM record with 8 MNK
+  int prof_as;
 };
 
 enum class Prec : int { F64 = 0, F32 = 1 };
@@ -74,14 +82,35 @@ hipError_t launch_transpose(int M, int N, int op, const cplx* S, int lds, long l
 // info (atomic min).
 constexpr int kLeaf = 64;
 constexpr int kMaxInverseN = 8192;
-size_t inverse_workspace_elems(int n, int batch);
+// Newton-Schulz iteration (newton = true, for drivers whose thread selected
+// SINV_INV_NEWTON with sinv_inverse_method): launches of kLeaf < n blocks
+// with at most kNewtonMaxBatch instances run X <- X (2I - M X) from the
+// Jacobi start instead of the elimination; instances whose residual does not
+// fall below the acceptance bound within the iteration limit are reported
+// through info with SINV_INFO_NOT_CONVERGED set in the key.  The workspace
+// must then come from inverse_workspace_elems(n, batch, true).
+constexpr int kNewtonMaxBatch = 4;
+int inverse_method();
+int newton_max_iter();
+size_t inverse_workspace_elems(int n, int batch, bool newton = false);
+// Newton-Schulz inverse of the nb = inner * outer <= kNewtonMaxBatch blocks M
+// into X (newton.hip); its buffers follow the elimination's `base` elements
+// of `work` (newton_workspace_elems(n) more).  in_place: M aliases X and is
+// copied aside first.
+size_t newton_workspace_elems(int n);
+hipError_t launch_newton_inverse(const cplx* M, int n, int ldm, long long sm_in, long long sm_out,
+                                 int inner, int outer, cplx* X, int ldx, long long sx_in,
+                                 long long sx_out, bool in_place, cplx* work, size_t base,
+                                 unsigned long long* info, int tag, hipStream_t s);
 hipError_t launch_inverse(cplx* A, int n, int lda, long long sA_in, long long sA_out, int inner,
-                          int outer, cplx* work, unsigned long long* info, int tag, hipStream_t s);
+                          int outer, cplx* work, unsigned long long* info, int tag, hipStream_t s,
+                          bool newton = false);
 // Out of place: A is destroyed (it is the elimination's working copy) and
 // its inverse is written to X.  Same workspace as launch_inverse.
 hipError_t launch_inverse_to(cplx* A, int n, int lda, long long sA_in, long long sA_out, int inner,
                              int outer, cplx* X, int ldx, long long sX_in, long long sX_out,
-                             cplx* work, unsigned long long* info, int tag, hipStream_t s);
+                             cplx* work, unsigned long long* info, int tag, hipStream_t s,
+                             bool newton = false);
 
 // Fill n elements with a value (used for NaN off-diagonals and info reset).
 hipError_t launch_fill(cplx* p, long long n, cplx v, hipStream_t s);
@@ -165,7 +194,8 @@ enum ProfKind : int {
   PROF_INV_LEAF = 2,
   PROF_OTHER = 3,
   PROF_GEMM_I8X = 4,
-  PROF_INV_PANEL = 5
+  PROF_INV_PANEL = 5,
+  PROF_INV_NEWTON = 6
 };
 int prof_open(hipStream_t s);
 void prof_close(int id, hipStream_t s, int kind, double flops, double bytes);

@@ -207,3 +207,36 @@ def test_inconsistent_shapes_raise_before_any_launch():
         core.zgemm_batched(A, np.zeros((2, 4, 6), complex))
     with pytest.raises(ValueError):
         core.zgemm_batched(A, B2, opA="X")
+
+
+def test_inverse_method_host_side(built_lib):
+    """The per-thread inverse method: argument checks, the previous method
+    returned, and workspace queries sized for the Newton-Schulz buffers only
+    while it is selected (host-only state, no GPU)."""
+    from sinv_amd import _native as nat
+    L = built_lib
+    assert L.sinv_inverse_method(5, 0) == -1
+    assert L.sinv_inverse_method(1, 17) == -1
+    assert L.sinv_inverse_method(1, -1) == -1
+    piv = L.sinv_zinv_workspace_bytes(512, 1)
+    with nat.inverse_method("newton", 3):
+        assert L.sinv_inverse_method(1, 0) == 1
+        n, m = 512, 4  # Y and the in-place copy of M for up to 4 blocks, 2 x 4 words
+        assert L.sinv_zinv_workspace_bytes(512, 1) == piv + (2 * n * n * m + 2 * m) * 16
+        assert L.sinv_zinv_workspace_bytes(64, 1) == 0
+        d = nat.RgfDesc(nblocks=8, blocksize=512, batch=1, precision=0,
+                        diag=ctypes.c_void_p(16), upper=ctypes.c_void_p(16),
+                        lower=ctypes.c_void_p(16), g_diag=ctypes.c_void_p(16),
+                        g_upper=ctypes.c_void_p(16), g_lower=ctypes.c_void_p(16), save_off_diag=1)
+        ws_newton = L.sinv_rgf_workspace_bytes(ctypes.byref(d))
+    assert L.sinv_inverse_method(0, 0) == 0
+    ws_piv = L.sinv_rgf_workspace_bytes(ctypes.byref(d))
+    assert ws_newton == ws_piv + (2 * 512 * 512 * 4 + 8) * 16
+    # a workspace sized under the elimination is refused under the iteration
+    with nat.inverse_method("newton"):
+        rc = L.sinv_rgf_batched(ctypes.byref(d), ctypes.c_void_p(256), ws_piv, None)
+    assert rc == nat.SINV_E_INVAL and b"workspace too small" in L.sinv_last_error()
+    with pytest.raises(ValueError):
+        nat.inverse_method("lu")
+    rc = L.sinv_column_dominance(None, None, None)
+    assert rc == nat.SINV_E_INVAL

@@ -102,27 +102,29 @@ def test_psr_seqsolve_dropin(built_lib, n, b):
 
 
 @pytest.mark.parametrize("P,first_rows", [(4, 0), (2, 13)])
-def test_psr_graph_capture_replays(built_lib, P, first_rows):
+@pytest.mark.parametrize("inverse", ["pivoted", "newton"])
+def test_psr_graph_capture_replays(built_lib, P, first_rows, inverse):
     """PSR forks its top and bottom partitions onto side streams and joins them
     back to the caller's stream.  Captured into a graph (after one eager call,
     as graph capture requires), every replay -- with eager work and
-    synchronisations in between -- equals the eager result bit for bit."""
+    synchronisations in between -- equals the eager result bit for bit, with
+    either block-inverse method (the iteration's state is reset by a kernel)."""
     import torch
     from sinv_amd import core
     from sinv_amd.utils import configs
-    print(f"psr graph P={P} first_rows={first_rows}", flush=True)
+    print(f"psr graph P={P} first_rows={first_rows} inverse={inverse}", flush=True)
     d, u, l = (core.to_dev(x) for x in configs.dominant_blocks(28, 96, seed=5))
-    ref = core.psr_batched(d, u, l, P, first_rows=first_rows)
+    ref = core.psr_batched(d, u, l, P, first_rows=first_rows, inverse=inverse)
     torch.cuda.synchronize()
     s = torch.cuda.Stream()
     s.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(s):
-        core.psr_batched(d, u, l, P, first_rows=first_rows, check=False)
+        core.psr_batched(d, u, l, P, first_rows=first_rows, check=False, inverse=inverse)
     torch.cuda.current_stream().wait_stream(s)
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g):
-        out = core.psr_batched(d, u, l, P, first_rows=first_rows, check=False)
+        out = core.psr_batched(d, u, l, P, first_rows=first_rows, check=False, inverse=inverse)
     x = torch.randn(512, 512, dtype=torch.float64, device=d.device)
     for rep in range(3):
         y = (x @ x).sum()  # eager work between replays

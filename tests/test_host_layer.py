@@ -194,3 +194,67 @@ def test_bcr_parallel_schedule_check_catches_a_gap():
             bcr_p.check_schedule_covers(15, 2)
     finally:
         bcr_p.reduction_schedule = orig
+
+
+def test_newton_iteration_count_bound():
+    """newton_iterations(rho) is the first step k with rho^(2^k) <= 1e-9 (the
+    acceptance bound; the 1-norm residual after k steps is at most rho^(2^k))."""
+    from sinv_amd import core
+    assert core.newton_iterations(1.0) is None and core.newton_iterations(1.5) is None
+    assert core.newton_iterations(float("nan")) is None
+    assert core.newton_iterations(0.0) == 1
+    for rho in np.linspace(0.001, 0.995, 200):
+        k = core.newton_iterations(rho)
+        assert rho ** (2 ** k) <= 1e-9 * (1 + 1e-12)
+        if k > 1:
+            assert rho ** (2 ** (k - 1)) > 1e-9
+
+
+def test_newton_schulz_on_dominant_schur_complements():
+    """The facts "auto" rests on, on the CPU: for a strictly column diagonally
+    dominant block-tridiagonal A, every Schur complement M_i that RGF inverts
+    (rgf.py:44-48, formed by the oracle's recurrence) has a column ratio no
+    larger than A's, and the iteration from X_1 = 2 D^-1 - D^-1 M D^-1 reaches
+    the acceptance bound within newton_iterations(rho_A) steps and then matches
+    numpy.linalg.inv.  A planted non-dominant block fails the first fact."""
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    d, u, l = configs.dominant_blocks(5, 48, seed=3)
+
+    def ratio(M):
+        a = np.abs(M)
+        dg = np.diag(a)
+        return ((a.sum(0) - dg) / dg).max()
+
+    b = 48
+    A = np.zeros((5 * b, 5 * b), complex)
+    for i in range(5):
+        A[i * b:(i + 1) * b, i * b:(i + 1) * b] = d[i]
+        if i < 4:
+            A[i * b:(i + 1) * b, (i + 1) * b:(i + 2) * b] = u[i]
+            A[(i + 1) * b:(i + 2) * b, i * b:(i + 1) * b] = l[i]
+    rho_a = ratio(A)
+    assert rho_a < 1
+    kmax = core.newton_iterations(rho_a)
+    g = None
+    for i in range(5):
+        M = d[i] if i == 0 else d[i] - l[i - 1] @ g @ u[i - 1]
+        assert ratio(M) <= rho_a
+        dg = np.diag(M)
+        X = 2 * np.diag(1 / dg) - (M / dg[:, None]) / dg[None, :]
+        # X_1 is one Newton-Schulz step from the Jacobi start D^-1
+        X0 = np.diag(1 / dg)
+        assert np.allclose(X, X0 @ (2 * np.eye(b) - M @ X0), rtol=1e-14, atol=0)
+        for k in range(1, kmax + 1):
+            T = 2 * np.eye(b) - M @ X
+            delta = np.abs(T - np.eye(b)).max()
+            X = X @ T
+            if delta <= 1e-9:
+                break
+        assert delta <= 1e-9
+        ref = np.linalg.inv(M)
+        assert np.linalg.norm(X - ref) / np.linalg.norm(ref) < 1e-13
+        g = X
+    # planted: a block with a weak diagonal breaks dominance (ratio above 1)
+    bad = d[0] - 2.9 * b * np.eye(b)
+    assert ratio(bad) > 1
