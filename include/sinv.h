@@ -355,8 +355,9 @@ int sinv_leaf_spin_limit(int polls);
 int sinv_inverse_method(int method, int max_iter);
 
 /* Column dominance ratio of block-tridiagonal instances described as for
- * sinv_rgf_batched (diag, upper, lower, strides, bsizes, batch, dscale,
- * oscale, shift; the output blocks are not used): rho[z] =
+ * sinv_rgf_batched with a uniform block size (diag, upper, lower, strides,
+ * batch, dscale, oscale, shift; bsizes must be NULL and the output blocks
+ * are not used): rho[z] =
  * max_j sum_{i != j} |a_ij| / |a_jj| over the whole matrix of instance z
  * (device array of batch doubles; +inf where a diagonal entry is zero).
  * rho < 1 means strictly column diagonally dominant, so every Schur

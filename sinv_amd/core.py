@@ -125,7 +125,7 @@ def newton_iterations(rho):
     return max(1, math.ceil(math.log2(math.log(NEWTON_ACCEPT) / math.log(rho))))
 
 
-def column_dominance(diag, upper, lower, *, shift=None, dscale=1.0, oscale=1.0, bsizes=None):
+def column_dominance(diag, upper, lower, *, shift=None, dscale=1.0, oscale=1.0):
     """Column dominance ratio max_j sum_{i != j} |a_ij| / |a_jj| of each
     block-tridiagonal instance (the RGF convention for diag / upper / lower,
     shift, dscale, oscale), as a float64 tensor on the device.  Below 1 the

@@ -199,60 +199,62 @@ __global__ __launch_bounds__(256) void ns_copy_kernel(const cplx* A, int n, int 
 }
 
 // Column dominance ratio of block column c of every instance: workgroup
-// (c, z), one thread per column (strided), rows summed down the column so
-// that a wave reads consecutive columns of each row.  rho holds the keys of
-// non-negative doubles (atomic maximum; cleared beforehand).
+// (column tile, z) takes 64 columns of one block column, its four waves
+// splitting the rows four ways (a wave reads 64 consecutive columns of a row),
+// partial sums combined through LDS.  rho holds the keys of non-negative
+// doubles (atomic maximum; cleared beforehand).
 __global__ __launch_bounds__(256) void dominance_kernel(
     const cplx* D, long long sd, const cplx* U, long long su, const cplx* L, long long sl,
-    int nblocks, int blocksize, int c_one, int bc_one, int bp_one, int bn_one, long long dof_one,
-    long long uof_one, long long lof_one, cplx dscale, cplx oscale, const cplx* shift,
+    int nblocks, int blocksize, cplx dscale, cplx oscale, const cplx* shift,
     unsigned long long* rho) {
-  // uniform block sizes (blocksize > 0): block column c = blockIdx.x; per-block
-  // sizes: one launch per column c_one with its sizes and offsets given
+  // blockIdx.x = c * tiles + column tile of block column c
   const int z = blockIdx.y;
-  const bool uni = blocksize > 0;
-  const int c = uni ? (int)blockIdx.x : c_one;
-  const long long b2 = uni ? (long long)blocksize * blocksize : 0;
-  const int bc = uni ? blocksize : bc_one;
-  const int bp = c > 0 ? (uni ? blocksize : bp_one) : 0;              // b_{c-1}
-  const int bn = c < nblocks - 1 ? (uni ? blocksize : bn_one) : 0;    // b_{c+1}
-  const long long dof = uni ? c * b2 : dof_one;
-  const long long uof = c > 0 ? (uni ? (c - 1) * b2 : uof_one) : 0;
-  const long long lof = c < nblocks - 1 ? (uni ? c * b2 : lof_one) : 0;
-  const cplx* Dc = D + z * sd + dof;
-  const cplx* Uc = U ? U + z * su + uof : nullptr;  // A_{c-1,c}: bp x bc
-  const cplx* Lc = L ? L + z * sl + lof : nullptr;  // A_{c+1,c}: bn x bc
+  const int bc = blocksize;
+  const int tiles = (bc + 63) / 64;
+  const int c = (int)blockIdx.x / tiles;
+  const int j0 = ((int)blockIdx.x % tiles) * 64;
+  const long long b2 = (long long)bc * bc;
+  const cplx* Dc = D + z * sd + c * b2;
+  // A_{c-1,c} = upper[c-1] and A_{c+1,c} = lower[c], where they exist
+  const cplx* Uc = c > 0 ? U + z * su + (c - 1) * b2 : nullptr;
+  const cplx* Lc = c < nblocks - 1 ? L + z * sl + c * b2 : nullptr;
   const double ds = sqrt(dscale.re * dscale.re + dscale.im * dscale.im);
   const double os = sqrt(oscale.re * oscale.re + oscale.im * oscale.im);
-  const cplx sh = shift ? shift[z] : cplx{0.0, 0.0};
-  unsigned long long m = 0ull;
-  for (int j = threadIdx.x; j < bc; j += 256) {
-    double off = 0.0;
-    for (int i = 0; i < bc; ++i) {
+  const int j = j0 + (int)(threadIdx.x & 63);
+  const int ph = (int)(threadIdx.x >> 6);
+  double off = 0.0, coup = 0.0;
+  if (j < bc) {
+    for (int i = ph; i < bc; i += 4) {
       if (i == j) continue;
       const cplx a = ld_c(Dc + (long long)i * bc + j);
-      off += ds * sqrt(a.re * a.re + a.im * a.im);
+      off += sqrt(a.re * a.re + a.im * a.im);
     }
-    double coup = 0.0;
-    if (bp > 0 && Uc)
-      for (int i = 0; i < bp; ++i) {
+    if (Uc)
+      for (int i = ph; i < bc; i += 4) {
         const cplx a = ld_c(Uc + (long long)i * bc + j);
         coup += sqrt(a.re * a.re + a.im * a.im);
       }
-    if (bn > 0 && Lc)
-      for (int i = 0; i < bn; ++i) {
+    if (Lc)
+      for (int i = ph; i < bc; i += 4) {
         const cplx a = ld_c(Lc + (long long)i * bc + j);
         coup += sqrt(a.re * a.re + a.im * a.im);
       }
-    off += os * coup;
+  }
+  __shared__ double part[4][64];
+  part[ph][threadIdx.x & 63] = ds * off + os * coup;
+  __syncthreads();
+  unsigned long long m = 0ull;
+  if (ph == 0 && j < bc) {
+    const double tot = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                       part[3][threadIdx.x];
+    const cplx sh = shift ? shift[z] : cplx{0.0, 0.0};
     const cplx dj = ld_c(Dc + (long long)j * bc + j);
     const double are = dscale.re * dj.re - dscale.im * dj.im + sh.re;
     const double aim = dscale.re * dj.im + dscale.im * dj.re + sh.im;
     const double ad = sqrt(are * are + aim * aim);
-    double ratio = ad > 0.0 ? off / ad : __builtin_inf();
+    double ratio = ad > 0.0 ? tot / ad : __builtin_inf();
     if (ratio != ratio) ratio = __builtin_inf();
-    const unsigned long long k = nonneg_key(ratio);
-    if (k > m) m = k;
+    m = nonneg_key(ratio);
   }
   m = block_umax(m);
   if (threadIdx.x == 0) atomicMax(rho + z, m);
@@ -375,7 +377,7 @@ extern "C" int sinv_inverse_method(int method, int max_iter) {
 
 extern "C" int sinv_column_dominance(const sinv_rgf_desc* d, double* rho, void* stream) {
   if (d == nullptr || rho == nullptr || d->nblocks <= 0 || d->batch <= 0 || d->diag == nullptr ||
-      (d->bsizes == nullptr && d->blocksize <= 0) ||
+      d->bsizes != nullptr || d->blocksize <= 0 ||
       (d->nblocks > 1 && (d->upper == nullptr || d->lower == nullptr)) || d->diag_stride < 0 ||
       d->upper_stride < 0 || d->lower_stride < 0) {
     set_error("sinv_column_dominance: invalid descriptor");
@@ -394,18 +396,10 @@ extern "C" int sinv_column_dominance(const sinv_rgf_desc* d, double* rho, void* 
   const cplx* Lo = (const cplx*)d->lower;
   const cplx ds{d->dscale.re, d->dscale.im}, os{d->oscale.re, d->oscale.im};
   const cplx* sh = (const cplx*)d->shift;
-  if (d->bsizes == nullptr) {
-    hipLaunchKernelGGL(dominance_kernel, dim3((unsigned)L.N, (unsigned)d->batch), dim3(256), 0, s,
-                       D, d->diag_stride, U, d->upper_stride, Lo, d->lower_stride, L.N,
-                       d->blocksize, 0, 0, 0, 0, 0LL, 0LL, 0LL, ds, os, sh, keys);
-  } else {
-    for (int c = 0; c < L.N; ++c)
-      hipLaunchKernelGGL(dominance_kernel, dim3(1u, (unsigned)d->batch), dim3(256), 0, s, D,
-                         d->diag_stride, U, d->upper_stride, Lo, d->lower_stride, L.N, 0, c,
-                         L.b[c], c > 0 ? L.b[c - 1] : 0, c < L.N - 1 ? L.b[c + 1] : 0, L.doff[c],
-                         c > 0 ? L.ooff[c - 1] : 0LL, c < L.N - 1 ? L.ooff[c] : 0LL, ds, os, sh,
-                         keys);
-  }
+  const unsigned tiles = (unsigned)((d->blocksize + 63) / 64);
+  hipLaunchKernelGGL(dominance_kernel, dim3((unsigned)L.N * tiles, (unsigned)d->batch), dim3(256),
+                     0, s, D, d->diag_stride, U, d->upper_stride, Lo, d->lower_stride, L.N,
+                     d->blocksize, ds, os, sh, keys);
   SINV_HIP_TRY(hipGetLastError());
   return SINV_OK;
 }

@@ -33,7 +33,9 @@ def _short(name):
             return short
     for key in ("gemm_i8x", "slice_piece_kernel", "piece_max_kernel", "inv_leaf_kernel", "gj_flow_kernel", "gj_step_kernel", "gj_panel_global_kernel",
                 "gj_panel_kernel", "gj_extract_kernel", "gj_copy_kernel", "gj_permute_kernel",
-                "axpby_shift_kernel", "transpose_kernel", "fill_u64_kernel", "fill_kernel"):
+                "axpby_shift_kernel", "transpose_kernel", "fill_u64_kernel", "fill_kernel",
+                "ns_init_kernel", "ns_check_kernel", "ns_finish_kernel", "ns_copy_kernel",
+                "dominance_kernel"):
         if key in name:
             return key
     return name.split("(")[0][-60:]

@@ -240,3 +240,7 @@ def test_inverse_method_host_side(built_lib):
         nat.inverse_method("lu")
     rc = L.sinv_column_dominance(None, None, None)
     assert rc == nat.SINV_E_INVAL
+    # per-block sizes are refused before anything is launched
+    sizes = (ctypes.c_int * 8)(*([512] * 8))
+    d.bsizes = sizes
+    assert L.sinv_column_dominance(ctypes.byref(d), ctypes.c_void_p(256), None) == nat.SINV_E_INVAL
