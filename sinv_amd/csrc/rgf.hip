@@ -8,6 +8,13 @@
 //                             G_{i,i+1} = G_{i+1,i}^T (sym) | -g_i A_{i,i+1} G_{i+1,i+1}
 //                             G_ii      = g_i + g_i A_{i,i+1} X_i
 //
+// The forward sweep's T_{i+1} = A_{i+1,i} g_i is kept (in the G_{i,i+1}
+// output slot, unused until the backward step that overwrites it), so X_i is
+// one product, G_{i+1,i+1} T_{i+1}: six block products per block instead of
+// seven (five with sym_mat).  The reference evaluates (G_{i+1,i+1} A_{i+1,i}) g_i;
+// the other association agrees to rounding.  Without saved off-diagonals
+// there is no slot, and X_i is formed as (G_{i+1,i+1} A_{i+1,i}) g_i.
+//
 // Precision SINV_PREC_MIXED_REFINED runs the forward Schur products in
 // complex64 and refines each g~_i = inv(M~_i) once in FP64 against M_i formed
 // in FP64: R = M_i g~_i - I, g_i = g~_i - g~_i R; the backward sweep is FP64.
@@ -191,6 +198,13 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
                                d->info, i, s, newton);
     return launch_inverse(Gd + L.doff[i], bi, bi, 0, L.dsize, 1, nb, winv, d->info, i, s, newton);
   };
+  // T_i = lower[i-1] g_{i-1}: in the G_{i-1,i} output slot (same element
+  // count, leading dimension b_{i-1}) when off-diagonals are saved
+  const bool keep_t = d->save_off_diag != 0 && N > 1;
+  auto t_dst = [&](int i, cplx*& dst, long long& sdst) {
+    dst = keep_t ? Gu + L.ooff[i - 1] : P;
+    sdst = keep_t ? L.osize : b2;
+  };
   {
     const int b0 = L.b[0];
     cplx* dst;
@@ -202,23 +216,27 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
   }
   for (int i = 1; i < N; ++i) {
     const int bi = L.b[i], bp = L.b[i - 1];
+    cplx* T;
+    long long sT;
+    t_dst(i, T, sT);
     // T = lower[i-1] g_{i-1}              (b_i x b_{i-1})
     SINV_HIP_TRY(gemm(bi, bp, bp, Lo + L.ooff[i - 1], bp, d->lower_stride, Gd + L.doff[i - 1], bp,
-                      L.dsize, P, bp, b2, one, nullptr, 0, 0, zero, nullptr, prec));
+                      L.dsize, T, bp, sT, one, nullptr, 0, 0, zero, nullptr, prec));
     // M = dscale D_ii + shift I - oscale^2 T upper[i-1]
     cplx* dst;
     long long sdst;
     m_dst(i, dst, sdst);
-    SINV_HIP_TRY(gemm(bi, bi, bp, P, bp, b2, U + L.ooff[i - 1], bi, d->upper_stride, dst, bi, sdst,
+    SINV_HIP_TRY(gemm(bi, bi, bp, T, bp, sT, U + L.ooff[i - 1], bi, d->upper_stride, dst, bi, sdst,
                       mosq, D + L.doff[i], bi, d->diag_stride, dscale, shift, prec));
     SINV_HIP_TRY(invert_m(i));
     if (refine) {
       // one FP64 refinement step of g~_i against M_i formed in FP64 (in Q,
       // free once the inverse has consumed it)
+      // (T in FP64 replaces the complex64 one, so the backward sweep reads FP64)
       cplx* gi = Gd + L.doff[i];
       SINV_HIP_TRY(gemm(bi, bp, bp, Lo + L.ooff[i - 1], bp, d->lower_stride, Gd + L.doff[i - 1],
-                        bp, L.dsize, P, bp, b2, one, nullptr, 0, 0, zero, nullptr, Prec::F64));
-      SINV_HIP_TRY(gemm(bi, bi, bp, P, bp, b2, U + L.ooff[i - 1], bi, d->upper_stride, Q, bi, b2,
+                        bp, L.dsize, T, bp, sT, one, nullptr, 0, 0, zero, nullptr, Prec::F64));
+      SINV_HIP_TRY(gemm(bi, bi, bp, T, bp, sT, U + L.ooff[i - 1], bi, d->upper_stride, Q, bi, b2,
                         mosq, D + L.doff[i], bi, d->diag_stride, dscale, shift, Prec::F64));
       // R = M_i g~_i - I                     (in Xw)
       SINV_HIP_TRY(gemm(bi, bi, bi, Q, bi, b2, gi, bi, L.dsize, Xw, bi, b2, one, nullptr, 0, 0,
@@ -237,12 +255,19 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
     const int bi = L.b[i], bj = L.b[i + 1];
     cplx* Xn = save ? Gl + L.ooff[i] : Xw;  // holds G_{i+1,i} = -X_i (b_{i+1} x b_i)
     const long long sX = save ? L.osize : b2;
-    // P = oscale G_{i+1,i+1} lower[i]     (b_{i+1} x b_i)
-    SINV_HIP_TRY(gemm(bj, bi, bj, Gd + L.doff[i + 1], bj, L.dsize, Lo + L.ooff[i], bi,
-                      d->lower_stride, P, bi, b2, oscale, nullptr, 0, 0, zero, nullptr, bprec));
-    // G_{i+1,i} = -P g_i
-    SINV_HIP_TRY(gemm(bj, bi, bi, P, bi, b2, Gd + L.doff[i], bi, L.dsize, Xn, bi, sX, mone,
-                      nullptr, 0, 0, zero, nullptr, bprec));
+    if (keep_t) {
+      // G_{i+1,i} = -oscale G_{i+1,i+1} T_{i+1}   (T_{i+1} in the G_{i,i+1} slot)
+      const cplx mos{-oscale.re, -oscale.im};
+      SINV_HIP_TRY(gemm(bj, bi, bj, Gd + L.doff[i + 1], bj, L.dsize, Gu + L.ooff[i], bi, L.osize,
+                        Xn, bi, sX, mos, nullptr, 0, 0, zero, nullptr, bprec));
+    } else {
+      // P = oscale G_{i+1,i+1} lower[i]     (b_{i+1} x b_i)
+      SINV_HIP_TRY(gemm(bj, bi, bj, Gd + L.doff[i + 1], bj, L.dsize, Lo + L.ooff[i], bi,
+                        d->lower_stride, P, bi, b2, oscale, nullptr, 0, 0, zero, nullptr, bprec));
+      // G_{i+1,i} = -P g_i
+      SINV_HIP_TRY(gemm(bj, bi, bi, P, bi, b2, Gd + L.doff[i], bi, L.dsize, Xn, bi, sX, mone,
+                        nullptr, 0, 0, zero, nullptr, bprec));
+    }
     // Q = oscale g_i upper[i]              (b_i x b_{i+1})
     SINV_HIP_TRY(gemm(bi, bj, bi, Gd + L.doff[i], bi, L.dsize, U + L.ooff[i], bj,
                       d->upper_stride, Q, bj, b2, oscale, nullptr, 0, 0, zero, nullptr, bprec));
