@@ -40,10 +40,11 @@ extern "C" {
 
 #define SINV_PREC_FP64 0  /* complex128 GEMMs on FP64 MFMA, FP64 inverses   */
 #define SINV_PREC_MIXED 1 /* complex64 GEMMs on FP32 MFMA, FP64 inverses    */
-#define SINV_PREC_INT8X 2 /* complex128 GEMMs from 7 int8 digit planes per   */
-                          /* operand on the int8 matrix cores (exact int32   */
-                          /* digit sums, error <= 2^-44 K max|row| max|col|  */
-                          /* per real product), FP64 inverses; taken by the  */
+#define SINV_PREC_INT8X 2 /* complex128 GEMMs from 6 balanced base-256 int8  */
+                          /* digit planes per operand on the int8 matrix     */
+                          /* cores (exact int32 digit sums, error <= 2^-42.6 */
+                          /* K max|row| max|col| per real product), FP64     */
+                          /* inverses; taken by the                          */
                           /* RGF and adjoint drivers; sinv_zgemm_batched     */
                           /* rejects it (sinv_zgemm_int8x_batched instead)   */
 #define SINV_PREC_MIXED_REFINED 3 /* RGF only: complex64 forward Schur GEMMs, FP64 */

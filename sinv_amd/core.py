@@ -208,7 +208,7 @@ def rgf_batched(diag, upper, lower, *, sym_mat=False, save_off_diag=True, precis
     precision (the same choices for every RGF entry point; the adjoint takes
     all but "mixed_refined"):
       "fp64"           complex128 throughout, the reference's arithmetic;
-      "int8x"          complex128 products formed from 7 int8 digit planes on the
+      "int8x"          complex128 products formed from 6 int8 digit planes on the
                        int8 matrix cores, FP64 inverses: within 1e-9 of "fp64"
                        on the benchmark instances;
       "mixed"          complex64 products, FP64 inverses: about 1e-7 relative
@@ -842,11 +842,13 @@ def zinv_batched(A, check=True, inverse="pivoted", max_iter=0):
 
 
 def zgemm_batched(A, B, *, opA="N", opB="N", alpha=1.0, beta=0.0, C0=None, shift=None,
-                  precision="fp64", slices=7):
+                  precision="fp64", slices=6):
     """C = alpha op(A) op(B) + beta C0 (+ shift I) over a batch (leading dim) of blocks.
 
-    precision "int8x" forms the complex128 product from `slices` int8 digit
-    planes per operand on the int8 matrix cores (sinv_zgemm_int8x_batched)."""
+    precision "int8x" forms the complex128 product from `slices` (5..7) balanced
+    base-256 int8 digit planes per operand on the int8 matrix cores
+    (sinv_zgemm_int8x_batched): error <= 4.1 (S + 4) 2^(-8S) K max|row| max|col|
+    per real product."""
     if opA not in nat.OPS or opB not in nat.OPS:
         raise ValueError(f"opA / opB must be one of {sorted(nat.OPS)}")
     if precision not in ("fp64", "mixed", "int8x"):

@@ -5,23 +5,24 @@
 // in and out, the same contract as launch_gemm.
 //
 // Method.  Each row r of op(A) (each column of op(B)) and each real part
-// p in {re, im, re + im} gets a power-of-two scale 2^e with max |x| < 2^e over
-// the row, and the scaled values x / 2^e in (-1, 1) are split into S signed
-// 7-bit digits by exact double arithmetic:
-//     y <- 128 y;  q_t = trunc(y);  y <- y - q_t     (t = 0 .. S-1)
-// so x = 2^e (sum_t q_t 2^(-7(t+1)) + rem), |rem| < 2^(-7S), |q_t| <= 127.
-// (The digits are taken as the base-128 digits of floor(|x / 2^e| 128^S) with
-// x's sign, which the sequential form yields.)  The digit products
-// q_t(A) q_u(B) summed over k are exact in int32 on
-// v_mfma_i32_16x16x64_i8; pairs with the same t + u share one accumulator, and
-// pairs with t + u > S - 1 are dropped.  After the k loop each accumulator is
-// converted to double and scaled by 2^(-7(t+u+2)) 2^(eA_r + eB_c).  Per term
-// the dropped pairs (S - 1 of size 2^(-7S) at most, then geometrically
-// smaller) and the two remainders beyond S digits give an error below
-// 4 (S + 1) 2^(-7S) max_k|a_rk| max_k|b_kc| (scales are under twice the
-// maxima), so one real dot product errs by at most that times K (S = 7:
-// 2^-44 K, typically far less as the dropped digits are not aligned); an
-// FP64 dot product has 2^-53 K sum|a||b| as its bound.  The complex
+// p in {re, im, re + im} gets a power-of-two scale 2^e with max |x| / 2^e <=
+// 127/128 over the row (the frexp exponent of the maximum, one higher when its
+// mantissa exceeds 127/128), and y = x / 2^e becomes the integer
+// N = trunc(y 2^(8S-1)), |N| <= 127 * 2^(8S-8), exact in double.  N is written
+// with S balanced base-256 digits, peeled from the low end (q = the low byte
+// taken in [-128, 127], N <- (N - q) / 256), which leaves a top digit in
+// [-127, 127]: x = 2^e (sum_t q_t 2^(-7-8t) + rem), |rem| < 2^(-(8S-1)).
+// The digit products q_t(A) q_u(B) summed over k are exact in int32 on
+// v_mfma_i32_16x16x64_i8 (|sum| <= S Kp 2^14 < 2^31); pairs with the same
+// t + u share one accumulator, and pairs with t + u > S - 1 are dropped.  After
+// the k loop each accumulator is converted to double and scaled by
+// 2^(-14-8(t+u)) 2^(eA_r + eB_c).  Per term the dropped pairs (S - 1 of size
+// 2^(-8S) at most, then geometrically smaller) and the two remainders beyond S
+// digits give an error below 4.1 (S + 4) 2^(-8S) max_k|a_rk| max_k|b_kc|
+// (scales are under 2 (128/127) times the maxima), so one real dot product
+// errs by at most that times K (S = 6: 2^-42.6 K, typically far less as the
+// dropped digits are not aligned); an FP64 dot product has 2^-53 K sum|a||b|
+// as its bound.  The complex
 // product uses the 3M form as the FP64 kernel does: T1 = Ar Br, T2 = Ai Bi,
 // T3 = (Ar + Ai)(Br + Bi), C = (T1 - T2) + i (T3 - T1 - T2).
 //
@@ -53,7 +54,7 @@ typedef int i4v __attribute__((ext_vector_type(4)));
 constexpr int kTile = 64;   // workgroup tile (rows and columns)
 constexpr int kChunk = 64;  // k per LDS chunk (one MFMA's k)
 constexpr int kRowB = 80;   // LDS row stride in bytes (64 + 16 pad: conflict-free b128 reads)
-constexpr int kMaxKp = 16384;  // S * Kp * 127^2 must stay below 2^31
+constexpr int kMaxKp = 16384;  // S * Kp * 2^14 must stay below 2^31 (S <= 7)
 
 inline long long round_up(long long x, long long m) { return (x + m - 1) / m * m; }
 
@@ -102,12 +103,14 @@ __device__ __forceinline__ const cplx* slice_src(const SliceArgs& a, long long z
   return a.X + z_in * a.s_in + z_out * a.s_out;
 }
 
-// scale of a row from its maximum magnitude: 2^e with max < 2^e (frexp gives
-// max = f 2^e, f in [0.5, 1)); inv = 2^-e.  An all-zero row gets 0 and 0.
+// scale of a row from its maximum magnitude: 2^e with max / 2^e <= 127/128
+// (frexp gives max = f 2^e, f in [0.5, 1); e + 1 when f > 127/128, so the top
+// digit stays in int8); inv = 2^-e.  An all-zero row gets 0 and 0.
 __device__ __forceinline__ void scale_of(double mx, double& sc, double& inv) {
   if (mx > 0.0) {
     int e;
-    (void)frexp(mx, &e);
+    const double f = frexp(mx, &e);
+    if (f > 127.0 / 128.0) ++e;
     sc = ldexp(1.0, e);
     inv = ldexp(1.0, -e);
   } else {
@@ -116,32 +119,23 @@ __device__ __forceinline__ void scale_of(double mx, double& sc, double& inv) {
   }
 }
 
-// S digits of y in (-1, 1), most significant first: the base-128 digits of
-// floor(|y| 128^S) carrying y's sign, which is what the sequential form
-//     y <- 128 y;  q_t = trunc(y);  y <- y - q_t
-// produces (trunc keeps every remainder on y's side of zero).  |y| 128^S is
-// exact in double (a power-of-two scaling) and below 2^(7S); it is split
-// exactly into a high part of 4 digits (28 bits) and a low part of S - 4
-// digits, each converted to a 32-bit integer once.
+// S balanced base-256 digits of y, |y| <= 127/128, most significant first:
+// N = trunc(y 2^(8S-1)) (a power-of-two scaling, exact, |N| < 2^47 for S <= 6
+// and < 2^55 for S = 7), then from the low end q = the low byte of N taken in
+// [-128, 127] and N <- (N - q) / 256 (exact: N - q is a multiple of 256); the
+// last N is the top digit, in [-127, 127].  Digit t weighs 2^(-7-8t).
 template <int S>
 __device__ __forceinline__ void digits(double y, int q[S]) {
-  static_assert(S >= 5 && S <= 8, "4 high digits plus 1..4 low digits");
-  constexpr int LB = 7 * (S - 4);
-  const double a = __builtin_ldexp(fabs(y), 7 * S);
-  const double hd = trunc(__builtin_ldexp(a, -LB));
-  const unsigned hi = (unsigned)hd;
-  const unsigned lo = (unsigned)(a - __builtin_ldexp(hd, LB));  // truncates: drops the remainder
-  const int neg = y < 0.0;
+  static_assert(S >= 2 && S <= 7, "the top integer must fit 56 bits");
+  long long n = (long long)trunc(__builtin_ldexp(y, 8 * S - 1));
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int d = (int)((hi >> (7 * (3 - t))) & 127u);
-    q[t] = neg ? -d : d;
+  for (int t = S - 1; t > 0; --t) {
+    const int r = (int)(n & 255);
+    const int d = r >= 128 ? r - 256 : r;
+    q[t] = d;
+    n = (n - d) >> 8;
   }
-#pragma unroll
-  for (int t = 4; t < S; ++t) {
-    const int d = (int)((lo >> (7 * (S - 1 - t))) & 127u);
-    q[t] = neg ? -d : d;
-  }
+  q[0] = (int)n;
 }
 
 // byte offset of digit t, part p, row r, k chunk kc (k%64 added by the caller)
@@ -402,7 +396,7 @@ __global__ __launch_bounds__(256) void gemm_i8x_kernel(GemmArgs g, ProdArgs o, i
           double val = 0.0;
 #pragma unroll
           for (int d = S - 1; d >= 0; --d)
-            val += (double)acc[rt][ct][d][v] * __builtin_ldexp(1.0, -7 * (d + 2));
+            val += (double)acc[rt][ct][d][v] * __builtin_ldexp(1.0, -14 - 8 * d);
           val *= sa[rt * 16 + (lane >> 4) * 4 + v] * scol;
           if (p < 2) cr[rt][ct][v] += sr * val;
           ci[rt][ct][v] += si * val;
@@ -538,7 +532,7 @@ void gemm_i8x_o2_kernel(GemmArgs g, ProdArgs o, int tiles_m, int tiles_n) {
           double val = 0.0;
 #pragma unroll
           for (int d = S - 1; d >= 0; --d)
-            val += (double)acc[rt][ct][d][v] * __builtin_ldexp(1.0, -7 * (d + 2));
+            val += (double)acc[rt][ct][d][v] * __builtin_ldexp(1.0, -14 - 8 * d);
           val *= sa[rt * 16 + (lane >> 4) * 4 + v] * scol;
           if (p < 2) cr[rt][ct][v] += sr * val;
           ci[rt][ct][v] += si * val;

@@ -52,9 +52,10 @@ enum class Prec : int { F64 = 0, F32 = 1 };
 hipError_t launch_gemm(const GemmArgs& g, Prec p, hipStream_t s);
 
 // K2 in FP64-accurate sliced-int8 form (int8x.hip): the same product from S
-// 7-bit digit planes per operand on the int8 matrix cores, exact int32 sums.
-// The workspace holds the digit planes and row scales of both operands.
-constexpr int kI8xMinSlices = 5, kI8xMaxSlices = 7, kI8xDefaultSlices = 7;
+// balanced base-256 digit planes per operand on the int8 matrix cores, exact
+// int32 sums.  The workspace holds the digit planes and row scales of both
+// operands.
+constexpr int kI8xMinSlices = 5, kI8xMaxSlices = 7, kI8xDefaultSlices = 6;
 bool i8x_slices_ok(int S);
 size_t gemm_i8x_workspace_bytes(const GemmArgs& g, int S);
 hipError_t launch_gemm_i8x(const GemmArgs& g, int S, void* work, size_t work_bytes,

@@ -4,7 +4,7 @@ energies inside the band), relative Frobenius over every selected block,
 against the FP64 path (itself held to the oracle's rounding floor in
 test_gpu_rgf.py).
 
-* int8x: complex128 GEMMs from 7 int8 digit planes: inside the 1e-8 gate.
+* int8x: complex128 GEMMs from 6 int8 digit planes: inside the 1e-8 gate.
 * mixed: complex64 GEMMs: outside the gate, by design (~1e-7 product rounding
   amplified by the conditioning of the Schur complements); the test holds it
   to its measured bound so that a regression still shows.

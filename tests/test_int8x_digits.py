@@ -1,98 +1,135 @@
 """CPU checks of the digit split behind the int8x GEMM (sinv_amd/csrc/int8x.hip).
 
-The kernel takes the S signed 7-bit digits of y in (-1, 1) as the base-128
-digits of floor(|y| 128^S) with y's sign, from a high part of 4 digits and a
-low part of S - 4 digits.  These tests restate both that form and the
-sequential form (y <- 128 y; q = trunc(y); y <- y - q) in numpy and check that
-they agree digit for digit, that every digit fits int8, and that the digits
-reconstruct y to within 128^-S; then that the digit-pair product with pairs of
-index sum >= S dropped stays inside the bound the GPU tests use.
+Each row (column) gets a power-of-two scale 2^e with |x| / 2^e <= 127/128, and
+y = x / 2^e becomes the integer N = trunc(y 2^(8S-1)), |N| <= 127 * 2^(8S-8),
+written with S balanced base-256 digits: the kernel peels them from the low
+end (d = the low byte taken in [-128, 127], N <- (N - d) / 256), which leaves a
+top digit in [-127, 127]; digit t weighs 2^(-7-8t).  These tests restate that
+peel and an independent top-down construction with Python integers, check that
+they agree digit for digit, that every digit fits int8, that the digits
+reconstruct y to within 2^(-(8S-1)) on y's side of zero, and that the
+digit-pair product with pairs of index sum >= S dropped stays inside the bound
+the GPU tests use.
 """
 
 import numpy as np
 import pytest
 
 
-def digits_sequential(y, S):
+def scale_exp(mx):
+    """Exponent e of the scale 2^e of a row whose largest magnitude is mx."""
+    f, e = np.frexp(mx)
+    return e + 1 if f > 127 / 128 else e
+
+
+def top_integer(y, S):
+    return np.trunc(np.ldexp(y, 8 * S - 1)).astype(np.int64)
+
+
+def digits_peel(y, S):
+    """The kernel's order: low byte balanced into [-128, 127], then shift."""
+    n = top_integer(y, S)
     q = np.zeros(y.shape + (S,), dtype=np.int64)
-    r = y.copy()
-    for t in range(S):
-        r = r * 128.0
-        d = np.trunc(r)
-        q[..., t] = d.astype(np.int64)
-        r = r - d
+    for t in range(S - 1, 0, -1):
+        r = n & 255
+        d = np.where(r >= 128, r - 256, r)
+        q[..., t] = d
+        n = (n - d) >> 8
+    q[..., 0] = n
     return q
 
 
-def digits_integer(y, S):
-    lb = 7 * (S - 4)
-    a = np.ldexp(np.abs(y), 7 * S)
-    hd = np.trunc(np.ldexp(a, -lb))
-    hi = hd.astype(np.uint64)
-    lo = np.trunc(a - np.ldexp(hd, lb)).astype(np.uint64)
-    q = np.zeros(y.shape + (S,), dtype=np.int64)
-    for t in range(4):
-        q[..., t] = ((hi >> np.uint64(7 * (3 - t))) & np.uint64(127)).astype(np.int64)
-    for t in range(4, S):
-        q[..., t] = ((lo >> np.uint64(7 * (S - 1 - t))) & np.uint64(127)).astype(np.int64)
-    return np.where((y < 0)[..., None], -q, q)
+def digits_topdown(y, S):
+    """Independent construction: the top digit first, as the smallest value
+    that leaves a remainder the S - 1 lower balanced digits can express."""
+    out = np.zeros(y.shape + (S,), dtype=np.int64)
+    for idx, v in np.ndenumerate(y):
+        n = int(np.trunc(np.ldexp(v, 8 * S - 1)))
+        for t in range(S):
+            w = 256 ** (S - 1 - t)
+            lo_max = 127 * (w - 1) // 255      # largest value of the lower digits
+            d = -((lo_max - n) // w)            # ceil((n - lo_max) / w)
+            out[idx + (t,)] = d
+            n -= d * w
+        assert n == 0
+    return out
 
 
 def _samples(rng, n):
-    y = rng.uniform(-1, 1, n)
-    # magnitudes across the whole exponent range below 1, exact digit boundaries and signed zeros
-    y = np.concatenate([y, rng.uniform(-1, 1, n) * 2.0 ** rng.integers(-60, 0, n),
-                        np.array([0.0, -0.0, 1 - 2 ** -53, -(1 - 2 ** -53), 2 ** -7, -2 ** -7,
-                                  127 / 128, -127 / 128, 2 ** -49, -2 ** -49, 2 ** -50])])
+    y = rng.uniform(-127 / 128, 127 / 128, n)
+    # magnitudes across the whole exponent range below the bound, byte
+    # boundaries of the integer, and signed zeros
+    y = np.concatenate([y, rng.uniform(-1, 1, n) * 2.0 ** rng.integers(-60, -1, n),
+                        np.array([0.0, -0.0, 127 / 128, -127 / 128, 2 ** -7, -2 ** -7,
+                                  0.5, -0.5, 2 ** -47, -2 ** -47, 2 ** -48, 1.5 * 2 ** -8,
+                                  -(1.5 * 2 ** -8), 127.5 / 256, -127.5 / 256])])
     return y
 
 
 @pytest.mark.parametrize("S", [5, 6, 7])
-def test_integer_digits_equal_sequential_digits(S):
-    y = _samples(np.random.default_rng(S), 20000)
-    a = digits_sequential(y, S)
-    b = digits_integer(y, S)
+def test_peeled_digits_equal_topdown_digits(S):
+    y = _samples(np.random.default_rng(S), 4000)
+    a = digits_peel(y, S)
+    b = digits_topdown(y, S)
     assert np.array_equal(a, b)
-    assert np.abs(a).max() <= 127
-    w = 128.0 ** -np.arange(1, S + 1)
+    assert a[..., 1:].min() >= -128 and a[..., 1:].max() <= 127
+    assert np.abs(a[..., 0]).max() <= 127
+    w = 2.0 ** (-7 - 8 * np.arange(S))
     rec = (a * w).sum(axis=-1)
-    assert np.all(np.abs(y - rec) < 128.0 ** -S)
-    # the remainder has y's sign (truncation toward zero)
+    assert np.all(np.abs(y - rec) < 2.0 ** -(8 * S - 1))
+    # the remainder has y's sign (the integer truncates toward zero)
     assert np.all((y - rec) * y >= 0)
 
 
-def test_planted_rounding_slip_is_caught():
-    """Rounding instead of truncating the first digit overflows int8 near 1 and
-    breaks the agreement: the check above would fail on such a slip."""
-    y = np.array([0.999, -0.9999, 0.5])
-    q = digits_sequential(y, 7)
-    slipped = q.copy()
-    slipped[:, 0] = np.rint(y * 128).astype(np.int64)
-    assert not np.array_equal(q, slipped)
-    assert np.abs(slipped).max() > 127
+def test_scale_keeps_the_top_digit_in_int8():
+    """Without the 127/128 rule a row maximum just below a power of two would
+    give a top digit of 128: the rule moves such rows to the next exponent."""
+    for mx in (1.0 - 2 ** -53, 0.999, 127.5 / 128, 0.75, 2.0 ** -30 * 0.9999):
+        e = scale_exp(mx)
+        y = np.array([mx, -mx]) / 2.0 ** e
+        assert np.all(np.abs(y) <= 127 / 128)
+        q = digits_peel(y, 6)
+        assert np.abs(q).max() <= 127
+    # planted: the frexp exponent alone overflows the top digit near 1
+    y = np.array([1.0 - 2 ** -53])
+    assert digits_peel(y, 6)[0, 0] == 128
 
 
-@pytest.mark.parametrize("S", [5, 7])
+def test_planted_unbalanced_digits_are_caught():
+    """Plain unsigned bytes (no balancing) reconstruct the same integer but
+    leave digits above 127, which the range check rejects."""
+    y = np.array([0.49, 0.99 * 127 / 128, 0.3 + 2 ** -30])
+    n = top_integer(np.abs(y), 6)
+    unbalanced = np.stack([(n >> (8 * (5 - t))) & 255 for t in range(6)], axis=-1)
+    assert unbalanced.max() > 127
+    assert not np.array_equal(unbalanced, digits_peel(np.abs(y), 6))
+
+
+@pytest.mark.parametrize("S", [5, 6, 7])
 def test_truncated_digit_product_inside_bound(S):
     """sum_k a_k b_k from row/column scales and digit pairs with t + u < S
     (exact integer sums, as on the int8 matrix cores), against the bound
-    4 (S + 1) 2^(-7S) K max|a| max|b| of the GPU tests."""
+    4.1 (S + 4) 2^(-8S) K max|a| max|b| of the GPU tests."""
     rng = np.random.default_rng(3)
     K = 200
+    worst = 0.0
     for _ in range(20):
         a = rng.normal(size=K) * 2.0 ** rng.integers(-30, 30)
         b = rng.normal(size=K) * 2.0 ** rng.integers(-30, 30)
-        ea = np.frexp(np.abs(a).max())[1]
-        eb = np.frexp(np.abs(b).max())[1]
-        qa = digits_integer(np.ldexp(a, -ea), S)
-        qb = digits_integer(np.ldexp(b, -eb), S)
+        ea = scale_exp(np.abs(a).max())
+        eb = scale_exp(np.abs(b).max())
+        qa = digits_peel(np.ldexp(a, -ea), S)
+        qb = digits_peel(np.ldexp(b, -eb), S)
         total = 0.0
-        for d in range(S):
+        for s in range(S):
             acc = 0
-            for t in range(d + 1):
-                acc += int(np.dot(qa[:, t], qb[:, d - t]))  # exact integer digit-pair sums
-            total += acc * 2.0 ** (-7 * (d + 2))
+            for t in range(s + 1):
+                acc += int(np.dot(qa[:, t], qb[:, s - t]))  # exact integer digit-pair sums
+            assert abs(acc) < 2 ** 31
+            total += acc * 2.0 ** (-14 - 8 * s)
         got = np.ldexp(total, ea + eb)
         exact = float(np.dot(a.astype(np.longdouble), b.astype(np.longdouble)))
-        bound = 4 * (S + 1) * 2.0 ** (-7 * S) * K * np.abs(a).max() * np.abs(b).max()
+        bound = 4.1 * (S + 4) * 2.0 ** (-8 * S) * K * np.abs(a).max() * np.abs(b).max()
         assert abs(got - exact) <= bound
+        worst = max(worst, abs(got - exact) / bound)
+    assert worst > 0  # the check measures something
