@@ -101,13 +101,17 @@ def _kind_summary(res, kind):
     if not kcnt:
         return None
     tf = kfl / (kms / 1e3) / 1e12 if kms > 0 else 0.0
-    d = dict(ms=round(kms, 3), launches=int(kcnt), avg_launch_us=round(kms / kcnt * 1e3, 3),
-             algorithmic_tflops=round(tf, 3))
+    d = dict(ms=round(kms, 3), launches=int(kcnt), avg_launch_us=round(kms / kcnt * 1e3, 3))
+    if kind == "i8x_split":
+        gbps = res["bytes"].get(kind, 0.0) / (kms / 1e3) / 1e9 if kms > 0 else 0.0
+        d["algorithmic_GBps"] = round(gbps, 1)
+        d["hbm_frac"] = round(gbps / (HBM_TBPS * 1e3), 4)
+        return d
+    d["algorithmic_tflops"] = round(tf, 3)
     if kind == "gemm_i8x":
         d["int8_tops_executed"] = round(tf * I8X_OPS_PER_FLOP, 1)
         d["int8_peak_frac"] = round(tf * I8X_OPS_PER_FLOP / PEAK_TFLOPS["gemm_i8x"], 4)
-        d["timed"] = ("each call's digit-split and product kernels together (event pair around the "
-                      "call), so the int8 rate is a lower bound for the product kernel alone")
+        d["timed"] = "the product kernel alone (the digit split is the i8x_split kind)"
     return d
 
 
@@ -207,6 +211,17 @@ def main():
         for p in plans:
             p.check()
         del plans
+        if args.workload == "energies":
+            # the int8x step too (its split and product kernels have their own names)
+            for g in groups:
+                g["out"] = None
+            plans = make_plans("int8x")
+            for p in plans:
+                p.run()
+            torch.cuda.synchronize()
+            for p in plans:
+                p.check()
+            del plans
         n = 1 << 28  # float64 elements: 2 GiB read + 2 GiB written by one elementwise kernel
         src = torch.ones(n, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
@@ -307,7 +322,7 @@ def main():
         f"int8x {err_i8x:.3e}")
 
     # ------------------------------------------------------------ FP64 vs the CPU oracle (sample)
-    err_fp64 = err_floor = err_sample = None
+    err_fp64 = err_floor = err_sample = err_i8x_oracle = None
     if not args.no_cpu:
         from oracle import rgf_ref
         g = groups[0]
@@ -317,6 +332,12 @@ def main():
         p.run()
         p.check()
         got = tuple(x[0].cpu().numpy() for x in p.outputs())
+        p8 = core.RgfPlan(*g["dev"], shift=sh, dscale=g["dscale"], oscale=g["oscale"],
+                          precision="int8x", inverse=args.inverse)
+        p8.run()
+        p8.check()
+        got8 = tuple(x[0].cpu().numpy() for x in p8.outputs())
+        del p8
         if g["shift"] is not None:
             d, u, l = configs.shifted(g["H"], g["shift"][0])
         else:
@@ -325,6 +346,8 @@ def main():
         num = sum(np.linalg.norm(a - r) ** 2 for a, r in zip(got, ref))
         den = sum(np.linalg.norm(r) ** 2 for r in ref)
         err_fp64 = float(np.sqrt(num / den))
+        err_i8x_oracle = float(np.sqrt(sum(np.linalg.norm(a - r) ** 2
+                                           for a, r in zip(got8, ref)) / den))
         # the instance's rounding floor: the oracle against itself with every
         # triple product associated the other way (an equally valid FP64 order)
         alt = rgf_ref.rgf_blocks(d, u, l, reassociate=True)
@@ -333,7 +356,7 @@ def main():
                       "relative Frobenius distance over every selected block; the floor is the oracle "
                       "against itself with each triple product associated the other way")
         log(f"[bench] fp64 GPU vs CPU oracle (b={g['b']}, first energy): {err_fp64:.3e} "
-            f"(oracle vs its reassociated self: {err_floor:.3e})")
+            f"(oracle vs its reassociated self: {err_floor:.3e}); int8x: {err_i8x_oracle:.3e}")
 
     # ------------------------------------------------------------ plain FP64, batch = 1 (sample)
     plain = None
@@ -495,8 +518,17 @@ def main():
                               ("mixed", PEAK_TFLOPS["gemm_f32"], 8.0)):
         t_min = (per * n_prod / (p_gemm * 1e12) + f_inv / (PEAK_TFLOPS["gemm_f64"] * 1e12)) * 1e3
         path_roof[prec] = dict(t_min_ms=round(t_min, 3), frac=round(t_min / results[prec]["ms"], 4))
+    # int8x: 3 real products x 21 digit pairs x 2 b^3 int8 operations per block
+    # product on the int8 matrix cores, the digit split's algorithmic bytes
+    # (profiled step) at the HBM peak, the inverses in FP64
+    i8_bytes = results["int8x"]["bytes"].get("i8x_split", 0.0)
+    t_min = (I8X_OPS_PER_FLOP * 8.0 * n_prod / (PEAK_TFLOPS["gemm_i8x"] * 1e12)
+             + i8_bytes / (HBM_TBPS * 1e12) + f_inv / (PEAK_TFLOPS["gemm_f64"] * 1e12)) * 1e3
+    path_roof["int8x"] = dict(t_min_ms=round(t_min, 3), frac=round(t_min / results["int8x"]["ms"], 4),
+                              split_bytes=i8_bytes)
     path_roof["work"] = dict(
         executed_gemm_flops_fp64=6.0 * n_prod, executed_gemm_flops_mixed=8.0 * n_prod,
+        executed_int8_ops=I8X_OPS_PER_FLOP * 8.0 * n_prod,
         inverse_flops=f_inv, reference_convention_flops=alg_flops,
         note="executed work: 6 (N-1) block products per solve (3M in FP64), N inverses at 8 b^3; "
              "the line's tflops count the reference's convention (8 (N-1) products of 8 b^3)")
@@ -551,7 +583,8 @@ def main():
             "int8x": "gemm_i8x"}[head]
     kms, kcnt, kfl = results[head]["prof"][kind]
     kname = {"fp64": "gemm_kernel<double>", "mixed": "gemm_kernel<float>",
-             "mixed_refined": "gemm_kernel<double>", "int8x": "gemm_i8x_kernel<7>"}[head]
+             "mixed_refined": "gemm_kernel<double>", "int8x": "gemm_i8x_o2_kernel<6>"}[head]
+    tkey = "gemm_i8x_o2_kernel" if head == "int8x" else kname  # name in the traffic summary
     achieved = kfl / (kms / 1e3) / 1e12 if kms > 0 else 0.0
     if head == "int8x":
         # executed int8 operations: 3 real products x 21 digit pairs x 2 MNK = 15.75 x 8 MNK
@@ -559,7 +592,7 @@ def main():
     roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_TFLOPS[kind],
                 unit="TOP/s" if head == "int8x" else "TFLOP/s",
                 frac=round(achieved / PEAK_TFLOPS[kind], 4),
-                traffic=measured_traffic(wl["workload"], kname),
+                traffic=measured_traffic(wl["workload"], tkey),
                 kernel=kname, launches=kcnt, avg_launch_us=round(kms / max(kcnt, 1) * 1e3, 3),
                 flops_per_launch=kfl / max(kcnt, 1),
                 algorithmic_bytes_per_launch=results[head]["bytes"][kind] / max(kcnt, 1),
@@ -608,8 +641,9 @@ def main():
                        "of each g_i against M_i formed in FP64, FP64 backward sweep"),
             "batched_int8x": dict(solves_per_s=round(i8x_sps, 3), ms_per_step=round(results["int8x"]["ms"], 3),
                                   max_rel_err_vs_fp64=err_i8x, passes_err_gate=err_i8x <= ERR_GATE,
-                                  digit_planes=6,
-                                  gemm_i8x=_kind_summary(results["int8x"], "gemm_i8x")),
+                                  err_vs_cpu_oracle=err_i8x_oracle, digit_planes=6,
+                                  gemm_i8x=_kind_summary(results["int8x"], "gemm_i8x"),
+                                  i8x_split=_kind_summary(results["int8x"], "i8x_split")),
             "plain_fp64_batch1": plain,
         },
         "speedup_vs_plain_fp64": (round(solves / (ms / 1e3) / plain["solves_per_s"], 2) if plain else None),
@@ -929,8 +963,12 @@ def bench_gemm(args):
                 core.zgemm_batched(A, H, precision=prec)
                 torch.cuda.synchronize()
             kind = "gemm_f64" if prec == "fp64" else "gemm_i8x"
+            call_ms = prof.stats[kind][0] + (prof.stats["i8x_split"][0] if prec == "int8x" else 0.0)
             row[prec] = dict(ms=round(ms, 4), tflops=round(8 * b ** 3 * nb / (ms / 1e3) / 1e12, 2),
-                             profiled_call_ms=round(prof.stats[kind][0], 4))
+                             profiled_call_ms=round(call_ms, 4))
+            if prec == "int8x":
+                row[prec]["product_ms"] = round(prof.stats["gemm_i8x"][0], 4)
+                row[prec]["split_ms"] = round(prof.stats["i8x_split"][0], 4)
         ref = out["fp64"]
         row["int8x"]["rel_err_vs_fp64"] = float((out["int8x"] - ref).norm() / ref.norm())
         row["int8x_over_fp64_time"] = round(row["int8x"]["ms"] / row["fp64"]["ms"], 3)

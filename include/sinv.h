@@ -468,12 +468,15 @@ int sinv_gemm_select(int M, int N, int K, long long batch, int opA, int opB, int
  * and leaf-inverse kernels is bracketed by hipEvents on its own stream.
  * sinv_prof_end synchronises those events and returns, per kind
  * (0 = FP64 GEMM, 1 = FP32 GEMM, 2 = leaf inverse, 3 = other, 4 = int8x
- * GEMM call, 5 = tall-panel elimination of the blocked inverse), the summed
- * kernel time in ms, the launch count, the algorithmic flops (8 M N K per
- * complex GEMM, 8 n^3 per inverse) and, when `bytes` is not NULL, the
- * algorithmic bytes (16 B per complex element of each operand read and each
- * result written once: M K + K N + M N, + M N more with a C0 term; 2 n^2 per
- * in-place inverse).  Process-wide state: use from one thread, around a
+ * product kernel, 5 = tall-panel elimination of the blocked inverse,
+ * 6 = Newton-Schulz products, 7 = int8x digit split), the summed kernel time
+ * in ms, the launch count, the algorithmic flops (8 M N K per complex GEMM,
+ * int8x product included; 8 n^3 per inverse) and, when `bytes` is not NULL,
+ * the algorithmic bytes (16 B per complex element of each operand read and
+ * each result written once: M K + K N + M N, + M N more with a C0 term; 2 n^2
+ * per in-place inverse; int8x: the split reads each distinct operand copy once
+ * and writes 3 S B of digits per padded element, the product reads those
+ * digits once and writes C).  Process-wide state: use from one thread, around a
  * region with at most `max_launches` launches.
  * ------------------------------------------------------------------------ */
 int sinv_prof_begin(int max_launches);

@@ -273,7 +273,8 @@ class inverse_method:
         return False
 
 
-PROF_KINDS = ("gemm_f64", "gemm_f32", "inv_leaf", "other", "gemm_i8x", "inv_panel", "inv_newton")
+PROF_KINDS = ("gemm_f64", "gemm_f32", "inv_leaf", "other", "gemm_i8x", "inv_panel", "inv_newton",
+              "i8x_split")
 
 # leaf inverse kernels (include/sinv.h SINV_LEAF_*)
 LEAF_VARIANTS = {"auto": 1, "lds": 0, "rows8": 2, "panel8": 3, "panel16": 4, "panel16f": 5,

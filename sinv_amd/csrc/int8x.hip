@@ -7,15 +7,17 @@
 // Method.  Each row r of op(A) (each column of op(B)) and each real part
 // p in {re, im, re + im} gets a power-of-two scale 2^e with max |x| / 2^e <=
 // 127/128 over the row (the frexp exponent of the maximum, one higher when its
-// mantissa exceeds 127/128), and y = x / 2^e becomes the integer
-// N = trunc(y 2^(8S-1)), |N| <= 127 * 2^(8S-8), exact in double.  N is written
-// with S balanced base-256 digits, peeled from the low end (q = the low byte
-// taken in [-128, 127], N <- (N - q) / 256), which leaves a top digit in
-// [-127, 127]: x = 2^e (sum_t q_t 2^(-7-8t) + rem), |rem| < 2^(-(8S-1)).
-// The digit products q_t(A) q_u(B) summed over k are exact in int32 on
-// v_mfma_i32_16x16x64_i8 (|sum| <= S Kp 2^14 < 2^31); pairs with the same
-// t + u share one accumulator, and pairs with t + u > S - 1 are dropped.  After
-// the k loop each accumulator is converted to double and scaled by
+// mantissa exceeds 127/128), and x becomes the integer
+// N = rint(x 2^(8S-1-e)), |N| <= 127 * 2^(8S-8), written with S balanced
+// base-256 digits: the low S - 1 in [-128, 127] and a top digit in
+// [-127, 127], x = 2^e (sum_t q_t 2^(-7-8t) + rem), |rem| <= 2^(-8S).  The
+// digits are the bytes of m = N + B_S, B_S = 128 (1 + 256 + ... +
+// 256^(S-2)): biasing each low digit by 128 makes it the byte q_t + 128 of m
+// (so the digit's own byte is that byte XOR 0x80), and the top byte of m is
+// the top digit.  The digit products q_t(A) q_u(B) summed over k are exact in
+// int32 on v_mfma_i32_16x16x64_i8 (|sum| <= S Kp 2^14 < 2^31); pairs with the
+// same t + u share one accumulator, and pairs with t + u > S - 1 are dropped.
+// After the k loop each accumulator is converted to double and scaled by
 // 2^(-14-8(t+u)) 2^(eA_r + eB_c).  Per term the dropped pairs (S - 1 of size
 // 2^(-8S) at most, then geometrically smaller) and the two remainders beyond S
 // digits give an error below 4.1 (S + 4) 2^(-8S) max_k|a_rk| max_k|b_kc|
@@ -26,15 +28,17 @@
 // product uses the 3M form as the FP64 kernel does: T1 = Ar Br, T2 = Ai Bi,
 // T3 = (Ar + Ai)(Br + Bi), C = (T1 - T2) + i (T3 - T1 - T2).
 //
-// Data flow per launch: per operand, a row-maximum pass and a digit pass over
-// 64 x 64 pieces write the digits of op(A) and op(B) (int8, blocked by 64-wide
-// k chunks, k and rows padded to multiples of 64 with zeros) and their scales
-// into the caller's workspace; an operand with a zero batch stride is sliced
-// once.  The product kernel stages 64 x 64 digit tiles of all S digits of
-// op(A) and op(B) through LDS, runs the three real products one after another
-// over k, and folds each into double accumulators.  Two versions: one LDS
-// buffer with two workgroups per CU (the default), and two LDS buffers with
-// one workgroup per CU (selectable; identical bits).
+// Data flow per launch: per operand, a split writes the digits of op(A) and
+// op(B) (int8, blocked by 64-wide k chunks, k and rows padded to multiples of
+// 64 with zeros) and their scales into the caller's workspace; an operand with
+// a zero batch stride is split once.  For K <= 512 one kernel reads each row
+// once (its maximum and its digits from the same registers); for longer rows a
+// row-maximum pass over 64 x 64 pieces precedes the digit pass.  The product
+// kernel stages 64 x 64 digit tiles of all S digits of op(A) and op(B) through
+// LDS, runs the three real products one after another over k, and folds each
+// into double accumulators.  Two versions: one LDS buffer with two workgroups
+// per CU (the default), and two LDS buffers with one workgroup per CU
+// (selectable; identical bits).
 //
 // int8 MFMA 16x16x64 operands: lane l supplies 16 consecutive k of row l % 16
 // (A) or column l % 16 (B) starting at k = 16 (l / 16).  Any assignment of k to
@@ -55,6 +59,7 @@ constexpr int kTile = 64;   // workgroup tile (rows and columns)
 constexpr int kChunk = 64;  // k per LDS chunk (one MFMA's k)
 constexpr int kRowB = 80;   // LDS row stride in bytes (64 + 16 pad: conflict-free b128 reads)
 constexpr int kMaxKp = 16384;  // S * Kp * 2^14 must stay below 2^31 (S <= 7)
+constexpr int kSplitMaxChunks = 8;  // single-pass split up to K = 512
 
 inline long long round_up(long long x, long long m) { return (x + m - 1) / m * m; }
 
@@ -74,8 +79,12 @@ __host__ __device__ inline long long zindex(int mode, int z_in, int z_out, int i
 // imaginary part negated when conj.  Digit t of part p of element (r, k) of
 // distinct copy zi is stored blocked by 64-wide k chunks, so that one chunk's
 // 64 x 64 digit tile is 4 KB of consecutive bytes:
-//   data + zi*3*S*Rp*Kp + (((p*nkc + k/64)*S + t)*Rp + r)*64 + k%64
-// (nkc = Kp/64); scales at scale + (zi*3 + p)*Rp + r (2^e, 0 for an all-zero
+//   data + zi*3*S*Rp*Kp + (((p*nkc + k/64)*S + t)*Rp + r)*64 + pos(k%64)
+// (nkc = Kp/64), where within a chunk k = 4 j + q (q < 4, j < 16) sits at byte
+// pos = 16 q + j: the split's threads hold k interleaved by 4 (so that a load
+// instruction reads runs of each row) and write 16 consecutive bytes.  Both
+// operands use the same order, and a dot product does not depend on the order
+// of its terms.  Scales at scale + (zi*3 + p)*Rp + r (2^e, 0 for an all-zero
 // row).
 struct SliceArgs {
   const cplx* X;
@@ -105,37 +114,67 @@ __device__ __forceinline__ const cplx* slice_src(const SliceArgs& a, long long z
 
 // scale of a row from its maximum magnitude: 2^e with max / 2^e <= 127/128
 // (frexp gives max = f 2^e, f in [0.5, 1); e + 1 when f > 127/128, so the top
-// digit stays in int8); inv = 2^-e.  An all-zero row gets 0 and 0.
-__device__ __forceinline__ void scale_of(double mx, double& sc, double& inv) {
-  if (mx > 0.0) {
-    int e;
-    const double f = frexp(mx, &e);
-    if (f > 127.0 / 128.0) ++e;
-    sc = ldexp(1.0, e);
-    inv = ldexp(1.0, -e);
-  } else {
+// digit stays in int8).  Returns the exponent shift 8S - 1 - e that makes the
+// row's integers; an all-zero row gets scale 0 and shift 0 (its digits are 0).
+template <int S>
+__device__ __forceinline__ int row_shift(double mx, double& sc) {
+  if (!(mx > 0.0)) {
     sc = 0.0;
-    inv = 0.0;
+    return 0;
+  }
+  int e;
+  const double f = frexp(mx, &e);
+  if (f > 127.0 / 128.0) ++e;
+  sc = ldexp(1.0, e);
+  return 8 * S - 1 - e;
+}
+
+// m = rint(x 2^sh) + B_S (the digits as bytes, see the method above).  For
+// S <= 6, |rint(x 2^sh)| < 2^47 and one addition does it: x 2^sh + (1.5 2^52 +
+// B_S) rounds to the nearest integer in [2^52, 2^53) (ties to even; B_S is
+// even), whose 52 significand bits are 2^51 + m, so the low 48 bits of the
+// double's bit pattern are m mod 2^48.  S = 7 (|N| < 2^55) goes through int64.
+template <int S>
+__device__ __forceinline__ unsigned long long biased_integer(double x, int sh) {
+  static_assert(S >= 2 && S <= 7, "the biased integer must fit 56 bits");
+  constexpr unsigned long long bias = 0x8080808080808080ull >> (8 * (9 - S));
+  if constexpr (S <= 6) {
+    constexpr double magic = 6755399441055744.0 + (double)bias;  // 1.5 * 2^52 + B_S
+    return (unsigned long long)__double_as_longlong(__builtin_ldexp(x, sh) + magic);
+  } else {
+    return (unsigned long long)((long long)rint(__builtin_ldexp(x, sh)) + (long long)bias);
   }
 }
 
-// S balanced base-256 digits of y, |y| <= 127/128, most significant first:
-// N = trunc(y 2^(8S-1)) (a power-of-two scaling, exact, |N| < 2^47 for S <= 6
-// and < 2^55 for S = 7), then from the low end q = the low byte of N taken in
-// [-128, 127] and N <- (N - q) / 256 (exact: N - q is a multiple of 256); the
-// last N is the top digit, in [-127, 127].  Digit t weighs 2^(-7-8t).
+// Digit planes of four consecutive elements: word t holds digit t (t = 0 the
+// top digit) of elements 0..3 in bytes 0..3; digit t is byte S - 1 - t of m,
+// XOR 0x80 for the biased low digits.  A 4 x 4 byte transpose per 32-bit half
+// of m with v_perm_b32 (perm(s0, s1, sel): result byte i is byte sel_i of the
+// eight bytes s1[0..3], s0[0..3]).
+__device__ __forceinline__ void transpose4(unsigned a, unsigned b, unsigned c, unsigned d,
+                                           unsigned out[4]) {
+  const unsigned ab01 = __builtin_amdgcn_perm(b, a, 0x05010400u);  // a0 b0 a1 b1
+  const unsigned ab23 = __builtin_amdgcn_perm(b, a, 0x07030602u);  // a2 b2 a3 b3
+  const unsigned cd01 = __builtin_amdgcn_perm(d, c, 0x05010400u);
+  const unsigned cd23 = __builtin_amdgcn_perm(d, c, 0x07030602u);
+  out[0] = __builtin_amdgcn_perm(cd01, ab01, 0x05040100u);  // a0 b0 c0 d0
+  out[1] = __builtin_amdgcn_perm(cd01, ab01, 0x07060302u);  // a1 b1 c1 d1
+  out[2] = __builtin_amdgcn_perm(cd23, ab23, 0x05040100u);
+  out[3] = __builtin_amdgcn_perm(cd23, ab23, 0x07060302u);
+}
+
 template <int S>
-__device__ __forceinline__ void digits(double y, int q[S]) {
-  static_assert(S >= 2 && S <= 7, "the top integer must fit 56 bits");
-  long long n = (long long)trunc(__builtin_ldexp(y, 8 * S - 1));
+__device__ __forceinline__ void plane_words(const unsigned long long m[4], unsigned w[S]) {
+  unsigned lo[4], hi[4];
+  transpose4((unsigned)m[0], (unsigned)m[1], (unsigned)m[2], (unsigned)m[3], lo);
+  transpose4((unsigned)(m[0] >> 32), (unsigned)(m[1] >> 32), (unsigned)(m[2] >> 32),
+             (unsigned)(m[3] >> 32), hi);
 #pragma unroll
-  for (int t = S - 1; t > 0; --t) {
-    const int r = (int)(n & 255);
-    const int d = r >= 128 ? r - 256 : r;
-    q[t] = d;
-    n = (n - d) >> 8;
+  for (int t = 0; t < S; ++t) {
+    const int byte = S - 1 - t;
+    const unsigned v = byte < 4 ? lo[byte] : hi[byte - 4];
+    w[t] = t > 0 ? v ^ 0x80808080u : v;
   }
-  q[0] = (int)n;
 }
 
 // byte offset of digit t, part p, row r, k chunk kc (k%64 added by the caller)
@@ -144,21 +183,110 @@ __host__ __device__ __forceinline__ size_t dig_off(int S, int nkc, int Rp, int p
   return ((((size_t)p * nkc + kc) * S + t) * Rp + r) * 64;
 }
 
-__device__ __forceinline__ unsigned pack4(int a, int b, int c, int d) {
-  return (unsigned)(a & 0xff) | ((unsigned)(b & 0xff) << 8) | ((unsigned)(c & 0xff) << 16) |
-         ((unsigned)(d & 0xff) << 24);
+// The digits of a thread's 16 elements k = 64 kc + 4 j + kq (j < 16) of row
+// r, with exponent shifts sh[p]: 16 consecutive bytes per plane at byte
+// 16 kq of the chunk's row (the in-chunk order of the layout above).
+template <int S>
+__device__ __forceinline__ void write_digits(const SliceArgs& a, long long zi, int nkc, int kc,
+                                             int kq, int r, const cplx x[16], const int sh[3]) {
+  int8_t* base = a.data + (size_t)zi * 3 * S * a.Rp * a.Kp + kq * 16;
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    unsigned words[4][S];
+#pragma unroll
+    for (int j4 = 0; j4 < 4; ++j4) {
+      unsigned long long m[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        double v[3];
+        parts_of(x[j4 * 4 + j], a.conj, v);
+        m[j] = biased_integer<S>(v[p], sh[p]);
+      }
+      plane_words<S>(m, words[j4]);
+    }
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      i4v v4;
+      v4.x = (int)words[0][t];
+      v4.y = (int)words[1][t];
+      v4.z = (int)words[2][t];
+      v4.w = (int)words[3][t];
+      *reinterpret_cast<i4v*>(base + dig_off(S, nkc, a.Rp, p, kc, t, r)) = v4;
+    }
+  }
 }
 
-// Two passes over 64 x 64 pieces of op(X) (rows 64 blockIdx.x + [0, 64),
-// k in 64 blockIdx.y + [0, 64)), grid (Rp / 64, Kp / 64, distinct copies),
-// 256 threads; thread (rl, kg) takes row 64 blockIdx.x + rl and the 16
-// consecutive k of group kg:
-//   KC = false (row-contiguous storage, rs == 1): rl = lane, kg = wave, so a
+__device__ __forceinline__ void row_max16(const cplx x[16], double conj, double mx[3]) {
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    double v[3];
+    parts_of(x[j], conj, v);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) mx[p] = fmax(mx[p], fabs(v[p]));
+  }
+}
+
+// Single pass for rows of at most 8 chunks (K <= 512): grid (Rp / 16,
+// distinct copies), 64 nkc threads; wave kc takes chunk kc of 16 rows, thread
+// (rl, kq) the 16 elements k = 64 kc + 4 j + kq of row 16 blockIdx.x + rl,
+// held in registers from the maximum to the digits:
+//   KC = true (k-contiguous storage, ks == 1): rl = lane / 4, kq = lane % 4,
+//     so each load instruction reads 64 consecutive bytes of 16 rows;
+//   KC = false (row-contiguous storage, rs == 1): rl = lane % 16,
+//     kq = lane / 16: 256 consecutive bytes (16 rows) at each of 4 k.
+// A row's maximum: over the thread's 16 elements, the row's 4 threads (lane
+// exchanges), then the waves (LDS).  A wave's digit stores cover 16 whole
+// 64-byte rows (1 KB) per plane.
+template <int S, bool KC>
+__global__ __launch_bounds__(64 * kSplitMaxChunks) void split_rows_kernel(SliceArgs a, int nkc) {
+  __shared__ double part[kSplitMaxChunks][3][16];
+  __shared__ int shift[3][16];
+  const int lane = threadIdx.x & 63, kc = threadIdx.x >> 6;
+  const int rl = KC ? lane >> 2 : lane & 15;
+  const int kq = KC ? lane & 3 : lane >> 4;
+  const int r = blockIdx.x * 16 + rl;
+  const long long zi = blockIdx.y;
+  const cplx* X = slice_src(a, zi);
+  const bool live = r < a.R;
+  cplx x[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int k = kc * 64 + 4 * j + kq;
+    x[j] = (live && k < a.K) ? X[(long long)r * a.rs + (long long)k * a.ks] : cplx{0.0, 0.0};
+  }
+  double mx[3] = {0.0, 0.0, 0.0};
+  row_max16(x, a.conj, mx);
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    mx[p] = fmax(mx[p], __shfl_xor(mx[p], KC ? 1 : 16));
+    mx[p] = fmax(mx[p], __shfl_xor(mx[p], KC ? 2 : 32));
+  }
+  if (kq == 0) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) part[kc][p][rl] = mx[p];
+  }
+  __syncthreads();
+  if (threadIdx.x < 48) {
+    const int p = threadIdx.x >> 4, row = threadIdx.x & 15;
+    double m = 0.0;
+    for (int w = 0; w < nkc; ++w) m = fmax(m, part[w][p][row]);
+    double sc;
+    shift[p][row] = row_shift<S>(m, sc);
+    a.scale[(zi * 3 + p) * a.Rp + blockIdx.x * 16 + row] = sc;
+  }
+  __syncthreads();
+  const int sh[3] = {shift[0][rl], shift[1][rl], shift[2][rl]};
+  write_digits<S>(a, zi, nkc, kc, kq, r, x, sh);
+}
+
+// Two passes for longer rows, over 64 x 64 pieces of op(X) (rows
+// 64 blockIdx.x + [0, 64), k in 64 blockIdx.y + [0, 64)), grid
+// (Rp / 64, Kp / 64, distinct copies), 256 threads; thread (rl, kq) takes row
+// 64 blockIdx.x + rl and the 16 elements k = 64 kc + 4 j + kq:
+//   KC = false (row-contiguous storage, rs == 1): rl = lane, kq = wave, so a
 //     wave's loads are runs along the rows;
-//   KC = true (k-contiguous storage, ks == 1): rl = tid / 4, kg = tid % 4, so
-//     four threads read one row's 64 consecutive elements.
-// Either way a thread's digits for one plane are 16 consecutive bytes of the
-// blocked layout, and a wave's stores cover whole 64-byte rows.
+//   KC = true (k-contiguous storage, ks == 1): rl = tid / 4, kq = tid % 4, so
+//     each load instruction reads 64 consecutive bytes of each of 64 rows.
 //  1. piece_max: the piece's row maxima per part -> pmax[(zi*nkc + kc)*3 + p][r]
 //  2. slice_piece: each row's scale from its nkc partial maxima, then the
 //     piece's digits.
@@ -168,42 +296,37 @@ __device__ __forceinline__ double* pmax_at(double* pmax, long long zi, int nkc, 
 }
 
 template <bool KC>
-__device__ __forceinline__ void piece_coords(int& rl, int& kg) {
+__device__ __forceinline__ void piece_coords(int& rl, int& kq) {
   const int tid = threadIdx.x;
   rl = KC ? tid >> 2 : tid & 63;
-  kg = KC ? tid & 3 : tid >> 6;
+  kq = KC ? tid & 3 : tid >> 6;
+}
+
+template <bool KC>
+__device__ __forceinline__ void load_piece(const SliceArgs& a, const cplx* X, int r, int kc,
+                                           int kq, cplx x[16]) {
+  const bool live = r < a.R;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int k = kc * 64 + 4 * j + kq;
+    x[j] = (live && k < a.K) ? X[(long long)r * a.rs + (long long)k * a.ks] : cplx{0.0, 0.0};
+  }
 }
 
 template <bool KC>
 __global__ __launch_bounds__(256) void piece_max_kernel(SliceArgs a, double* pmax, int nkc) {
   __shared__ double part[4][3][64];
-  int rl, kg;
-  piece_coords<KC>(rl, kg);
+  int rl, kq;
+  piece_coords<KC>(rl, kq);
   const int r = blockIdx.x * 64 + rl;
   const int kc = blockIdx.y;
   const long long zi = blockIdx.z;
-  const cplx* X = slice_src(a, zi);
-  // a maximum does not care which k a thread sees: with k-contiguous storage
-  // the row's four threads interleave (k = 4 j + kg), so each load instruction
-  // reads 64 consecutive bytes of every row it touches
+  cplx x[16];
+  load_piece<KC>(a, slice_src(a, zi), r, kc, kq, x);
   double mx[3] = {0.0, 0.0, 0.0};
-  if (r < a.R) {
-    cplx x[16];
+  row_max16(x, a.conj, mx);
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const int k = kc * 64 + (KC ? 4 * j + kg : kg * 16 + j);
-      x[j] = k < a.K ? X[(long long)r * a.rs + (long long)k * a.ks] : cplx{0.0, 0.0};
-    }
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      double v[3];
-      parts_of(x[j], a.conj, v);
-#pragma unroll
-      for (int p = 0; p < 3; ++p) mx[p] = fmax(mx[p], fabs(v[p]));
-    }
-  }
-#pragma unroll
-  for (int p = 0; p < 3; ++p) part[kg][p][rl] = mx[p];
+  for (int p = 0; p < 3; ++p) part[kq][p][rl] = mx[p];
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (w < 3) {
@@ -216,56 +339,24 @@ __global__ __launch_bounds__(256) void piece_max_kernel(SliceArgs a, double* pma
 template <int S, bool KC>
 __global__ __launch_bounds__(256) void slice_piece_kernel(SliceArgs a, const double* pmax,
                                                           int nkc) {
-  int rl, kg_in;
-  piece_coords<KC>(rl, kg_in);
+  int rl, kq;
+  piece_coords<KC>(rl, kq);
   const int r = blockIdx.x * 64 + rl;
   const int kc = blockIdx.y;
   const long long zi = blockIdx.z;
-  const cplx* X = slice_src(a, zi);
-  double inv[3];
+  int sh[3];
 #pragma unroll
   for (int p = 0; p < 3; ++p) {
     double m = 0.0;
     for (int c = 0; c < nkc; ++c)
       m = fmax(m, pmax_at(const_cast<double*>(pmax), zi, nkc, c, p, a.Rp)[r]);
     double sc;
-    scale_of(m, sc, inv[p]);
-    if (kc == 0 && kg_in == 0) a.scale[(zi * 3 + p) * a.Rp + r] = sc;
+    sh[p] = row_shift<S>(m, sc);
+    if (kc == 0 && kq == 0) a.scale[(zi * 3 + p) * a.Rp + r] = sc;
   }
-  const bool live = r < a.R;
-  const int kg = kc * 4 + kg_in;  // 16-wide k group
   cplx x[16];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int k = kg * 16 + j;
-    x[j] = (live && k < a.K) ? X[(long long)r * a.rs + (long long)k * a.ks] : cplx{0.0, 0.0};
-  }
-  int8_t* base = a.data + (size_t)zi * 3 * S * a.Rp * a.Kp + kg_in * 16;
-#pragma unroll
-  for (int p = 0; p < 3; ++p) {
-    unsigned words[S][4];
-#pragma unroll
-    for (int j4 = 0; j4 < 4; ++j4) {
-      int q[4][S];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        double v[3];
-        parts_of(x[j4 * 4 + j], a.conj, v);
-        digits<S>(v[p] * inv[p], q[j]);
-      }
-#pragma unroll
-      for (int t = 0; t < S; ++t) words[t][j4] = pack4(q[0][t], q[1][t], q[2][t], q[3][t]);
-    }
-#pragma unroll
-    for (int t = 0; t < S; ++t) {
-      i4v v4;
-      v4.x = (int)words[t][0];
-      v4.y = (int)words[t][1];
-      v4.z = (int)words[t][2];
-      v4.w = (int)words[t][3];
-      *reinterpret_cast<i4v*>(base + dig_off(S, nkc, a.Rp, p, kc, t, r)) = v4;
-    }
-  }
+  load_piece<KC>(a, slice_src(a, zi), r, kc, kq, x);
+  write_digits<S>(a, zi, nkc, kc, kq, r, x, sh);
 }
 
 struct ProdArgs {
@@ -599,17 +690,19 @@ I8xPlan plan_i8x(const GemmArgs& g, int S) {
   off = al256(off + (size_t)p.b_nz * 3 * S * p.Np * p.Kp);
   p.b_scale = off;
   off = al256(off + (size_t)p.b_nz * 3 * p.Np * sizeof(double));
-  // partial row maxima of a row-contiguous operand (one per 64-wide k chunk)
+  // partial row maxima (one per 64-wide k chunk) of the two-pass split
   const size_t nkc = (size_t)p.Kp / kChunk;
   const size_t pa = (size_t)p.a_nz * nkc * 3 * p.Mp, pb = (size_t)p.b_nz * nkc * 3 * p.Np;
   p.pmax = off;
-  off = al256(off + (pa > pb ? pa : pb) * sizeof(double));
+  if (nkc > (size_t)kSplitMaxChunks) off = al256(off + (pa > pb ? pa : pb) * sizeof(double));
   p.total = off + 256;  // room to align the base
   return p;
 }
 
 template <int S>
 hipError_t launch_i8x_s(const GemmArgs& g, const I8xPlan& p, uint8_t* base, hipStream_t s) {
+  // the launch profiler times the digit split and the product separately
+  int pid = prof_open(s);
   // op(A): rows m, k along K;  element (m, k) = A[m*lda + k] (N) or A[k*lda + m] (T, C)
   SliceArgs sa{};
   sa.X = g.A; sa.s_in = g.sA_in; sa.s_out = g.sA_out; sa.inner = g.inner; sa.mode = p.a_mode;
@@ -631,10 +724,20 @@ hipError_t launch_i8x_s(const GemmArgs& g, const I8xPlan& p, uint8_t* base, hipS
   double* pmax = reinterpret_cast<double*>(base + p.pmax);
   for (const SliceArgs* a : {&sa, &sb}) {
     const unsigned nz = (unsigned)(a == &sa ? p.a_nz : p.b_nz);
+    const bool kc_major = a->ks == 1;
+    if (nkc <= kSplitMaxChunks) {
+      const dim3 grid((unsigned)(a->Rp / 16), nz);
+      if (kc_major)
+        hipLaunchKernelGGL((split_rows_kernel<S, true>), grid, dim3(64 * nkc), 0, s, *a, nkc);
+      else
+        hipLaunchKernelGGL((split_rows_kernel<S, false>), grid, dim3(64 * nkc), 0, s, *a, nkc);
+      SINV_HIP_RET(hipGetLastError());
+      continue;
+    }
     // the two operands' passes run one after the other on the stream, so they
     // share the partial-maximum buffer
     const dim3 grid((unsigned)(a->Rp / 64), (unsigned)nkc, nz);
-    if (a->ks == 1) {
+    if (kc_major) {
       hipLaunchKernelGGL(piece_max_kernel<true>, grid, dim3(256), 0, s, *a, pmax, nkc);
       SINV_HIP_RET(hipGetLastError());
       hipLaunchKernelGGL((slice_piece_kernel<S, true>), grid, dim3(256), 0, s, *a, pmax, nkc);
@@ -645,6 +748,16 @@ hipError_t launch_i8x_s(const GemmArgs& g, const I8xPlan& p, uint8_t* base, hipS
     }
     SINV_HIP_RET(hipGetLastError());
   }
+
+  {
+    // split: each distinct operand copy read once (16 B per complex element),
+    // its digits (3 S B per padded element) and scales written
+    const double rd = 16.0 * ((double)p.a_nz * g.M * g.K + (double)p.b_nz * g.N * g.K);
+    const double wr = 3.0 * S * ((double)p.a_nz * p.Mp + (double)p.b_nz * p.Np) * p.Kp +
+                      24.0 * ((double)p.a_nz * p.Mp + (double)p.b_nz * p.Np);
+    prof_close(pid, s, PROF_I8X_SPLIT, 0.0, rd + wr);
+  }
+  pid = prof_open(s);
 
   ProdArgs o{};
   o.a_data = sa.data; o.a_scale = sa.scale; o.a_mode = p.a_mode; o.a_Rp = p.Mp;
@@ -658,7 +771,14 @@ hipError_t launch_i8x_s(const GemmArgs& g, const I8xPlan& p, uint8_t* base, hipS
   else
     hipLaunchKernelGGL(gemm_i8x_o2_kernel<S>, dim3((unsigned)blocks), dim3(256), 0, s, g, o, tm,
                        tn);
-  return hipGetLastError();
+  SINV_HIP_RET(hipGetLastError());
+  // product: the digit planes read once, C written (and C0 read) once
+  const double nb = (double)g.inner * g.outer;
+  const double digits = 3.0 * S * ((double)p.a_nz * p.Mp + (double)p.b_nz * p.Np) * p.Kp;
+  prof_shape(pid, g.M, g.N, g.K, (long long)nb, g.opA | (g.opB << 2) | (g.C0 != nullptr ? 16 : 0));
+  prof_close(pid, s, PROF_GEMM_I8X, 8.0 * g.M * (double)g.N * g.K * nb,
+             digits + sizeof(cplx) * (double)g.M * g.N * nb * (g.C0 != nullptr ? 2.0 : 1.0));
+  return hipSuccess;
 }
 
 }  // namespace
@@ -683,20 +803,11 @@ hipError_t launch_gemm_i8x(const GemmArgs& g, int S, void* work, size_t work_byt
   if (blocks > 0x7fffffffLL || p.a_nz > 65535 || p.b_nz > 65535)
     return hipErrorInvalidConfiguration;
   uint8_t* base = (uint8_t*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
-  const int pid = prof_open(s);
-  hipError_t e;
   switch (S) {
-    case 5: e = launch_i8x_s<5>(g, p, base, s); break;
-    case 6: e = launch_i8x_s<6>(g, p, base, s); break;
-    default: e = launch_i8x_s<7>(g, p, base, s); break;
+    case 5: return launch_i8x_s<5>(g, p, base, s);
+    case 6: return launch_i8x_s<6>(g, p, base, s);
+    default: return launch_i8x_s<7>(g, p, base, s);
   }
-  if (e != hipSuccess) return e;
-  const double nb = (double)g.inner * g.outer;
-  prof_shape(pid, g.M, g.N, g.K, (long long)nb, g.opA | (g.opB << 2) | (g.C0 != nullptr ? 16 : 0));
-  const double elems = (double)g.M * g.K + (double)g.K * g.N +
-                       (double)g.M * g.N * (g.C0 != nullptr ? 2.0 : 1.0);
-  prof_close(pid, s, PROF_GEMM_I8X, 8.0 * g.M * (double)g.N * g.K * nb, sizeof(cplx) * elems * nb);
-  return hipSuccess;
 }
 
 }  // namespace sinv

@@ -196,7 +196,8 @@ enum ProfKind : int {
   PROF_OTHER = 3,
   PROF_GEMM_I8X = 4,
   PROF_INV_PANEL = 5,
-  PROF_INV_NEWTON = 6
+  PROF_INV_NEWTON = 6,
+  PROF_I8X_SPLIT = 7
 };
 int prof_open(hipStream_t s);
 void prof_close(int id, hipStream_t s, int kind, double flops, double bytes);

@@ -31,7 +31,8 @@ def _short(name):
                        ("gemm_kernel<float", "gemm_kernel<float>")):
         if key in name:
             return short
-    for key in ("gemm_i8x", "slice_piece_kernel", "piece_max_kernel", "inv_leaf_kernel", "gj_flow_kernel", "gj_step_kernel", "gj_panel_global_kernel",
+    for key in ("gemm_i8x_o2_kernel", "gemm_i8x_kernel", "split_rows_kernel",
+                "slice_piece_kernel", "piece_max_kernel", "inv_leaf_kernel", "gj_flow_kernel", "gj_step_kernel", "gj_panel_global_kernel",
                 "gj_panel_kernel", "gj_extract_kernel", "gj_copy_kernel", "gj_permute_kernel",
                 "axpby_shift_kernel", "transpose_kernel", "fill_u64_kernel", "fill_kernel",
                 "ns_init_kernel", "ns_check_kernel", "ns_finish_kernel", "ns_copy_kernel",

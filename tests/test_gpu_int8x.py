@@ -37,8 +37,9 @@ def _bound(A, B, opA, opB, S):
     return 3 * (4.1 * (S + 4) * 2.0 ** (-8 * S) + 1e-15) * K * ra * cb
 
 
+# K <= 512: single-pass digit split; K > 512: row-maximum pass + digit pass
 @pytest.mark.parametrize("M,N,K", [(64, 64, 64), (100, 70, 130), (1, 5, 3), (256, 128, 384),
-                                   (416, 256, 416), (33, 200, 65)])
+                                   (416, 256, 416), (33, 200, 65), (64, 96, 512), (40, 48, 700)])
 @pytest.mark.parametrize("ops", [("N", "N"), ("T", "N"), ("N", "C"), ("C", "T")])
 def test_zgemm_int8x_against_fp64(built_lib, M, N, K, ops):
     import torch
@@ -61,8 +62,10 @@ def test_zgemm_int8x_against_fp64(built_lib, M, N, K, ops):
         err = np.abs(C[z] - ref)
         bound = abs(alpha) * _bound(A[z], B[0], opA, opB, 6) + 1e-14 * np.abs(ref)
         assert np.all(err <= bound), (z, float((err / bound).max()))
+        # observed scale (the elementwise bound above is the guarantee): the
+        # row maxima, and with them the digits' quantum, grow slowly with K
         rel = np.linalg.norm(C[z] - ref) / np.linalg.norm(ref)
-        assert rel < 1e-13, rel
+        assert rel < 2e-13, rel
 
 
 @pytest.mark.parametrize("S", [5, 6, 7])

@@ -1,15 +1,18 @@
 """CPU checks of the digit split behind the int8x GEMM (sinv_amd/csrc/int8x.hip).
 
 Each row (column) gets a power-of-two scale 2^e with |x| / 2^e <= 127/128, and
-y = x / 2^e becomes the integer N = trunc(y 2^(8S-1)), |N| <= 127 * 2^(8S-8),
-written with S balanced base-256 digits: the kernel peels them from the low
-end (d = the low byte taken in [-128, 127], N <- (N - d) / 256), which leaves a
-top digit in [-127, 127]; digit t weighs 2^(-7-8t).  These tests restate that
-peel and an independent top-down construction with Python integers, check that
-they agree digit for digit, that every digit fits int8, that the digits
-reconstruct y to within 2^(-(8S-1)) on y's side of zero, and that the
-digit-pair product with pairs of index sum >= S dropped stays inside the bound
-the GPU tests use.
+y = x / 2^e becomes the integer N = rint(y 2^(8S-1)) (ties to even),
+|N| <= 127 * 2^(8S-8), written with S balanced base-256 digits: S - 1 low
+digits in [-128, 127] and a top digit in [-127, 127]; digit t weighs
+2^(-7-8t).  The kernel reads the digits off the bytes of m = N + B_S
+(B_S = 128 in each of the S - 1 low bytes: a biased low digit is its byte, the
+top digit is the top byte), and for S <= 6 forms m with one floating-point
+addition, y 2^(8S-1) + (1.5 2^52 + B_S), whose low 48 bits are m mod 2^48.
+These tests restate both steps, check them against two independent
+constructions of the balanced digits (peeling from the low end, and top-down
+with Python integers), that every digit fits int8, that the digits reconstruct
+y to within 2^(-8S), and that the digit-pair product with pairs of index sum
+>= S dropped stays inside the bound the GPU tests use.
 """
 
 import numpy as np
@@ -23,7 +26,30 @@ def scale_exp(mx):
 
 
 def top_integer(y, S):
-    return np.trunc(np.ldexp(y, 8 * S - 1)).astype(np.int64)
+    return np.rint(np.ldexp(y, 8 * S - 1)).astype(np.int64)
+
+
+def bias(S):
+    """128 in each of the S - 1 low bytes."""
+    return sum(128 << (8 * t) for t in range(S - 1))
+
+
+def digits_bytes(y, S):
+    """The kernel's method: the bytes of m = N + B_S (byte S - 1 - t is digit
+    t, minus 128 for the biased low digits)."""
+    m = top_integer(y, S) + bias(S)
+    q = np.zeros(y.shape + (S,), dtype=np.int64)
+    for t in range(S):
+        byte = (m >> (8 * (S - 1 - t))) & 255
+        q[..., t] = byte - 128 if t > 0 else np.where(byte >= 128, byte - 256, byte)
+    return q
+
+
+def magic_low_bits(y, S):
+    """For S <= 6: the low 48 bits of y 2^(8S-1) + (1.5 2^52 + B_S) as a double."""
+    magic = 1.5 * 2.0 ** 52 + bias(S)
+    t = np.ldexp(y, 8 * S - 1) + magic
+    return t.view(np.int64) & ((1 << 48) - 1)
 
 
 def digits_peel(y, S):
@@ -44,7 +70,7 @@ def digits_topdown(y, S):
     that leaves a remainder the S - 1 lower balanced digits can express."""
     out = np.zeros(y.shape + (S,), dtype=np.int64)
     for idx, v in np.ndenumerate(y):
-        n = int(np.trunc(np.ldexp(v, 8 * S - 1)))
+        n = int(np.rint(np.ldexp(v, 8 * S - 1)))
         for t in range(S):
             w = 256 ** (S - 1 - t)
             lo_max = 127 * (w - 1) // 255      # largest value of the lower digits
@@ -67,18 +93,32 @@ def _samples(rng, n):
 
 
 @pytest.mark.parametrize("S", [5, 6, 7])
-def test_peeled_digits_equal_topdown_digits(S):
+def test_byte_digits_equal_peeled_and_topdown_digits(S):
     y = _samples(np.random.default_rng(S), 4000)
-    a = digits_peel(y, S)
-    b = digits_topdown(y, S)
-    assert np.array_equal(a, b)
+    a = digits_bytes(y, S)
+    assert np.array_equal(a, digits_peel(y, S))
+    assert np.array_equal(a, digits_topdown(y, S))
     assert a[..., 1:].min() >= -128 and a[..., 1:].max() <= 127
     assert np.abs(a[..., 0]).max() <= 127
     w = 2.0 ** (-7 - 8 * np.arange(S))
     rec = (a * w).sum(axis=-1)
-    assert np.all(np.abs(y - rec) < 2.0 ** -(8 * S - 1))
-    # the remainder has y's sign (the integer truncates toward zero)
-    assert np.all((y - rec) * y >= 0)
+    # rint: the remainder is at most half of the last digit's weight
+    assert np.all(np.abs(y - rec) <= 2.0 ** -(8 * S))
+
+
+@pytest.mark.parametrize("S", [5, 6])
+def test_magic_addition_gives_the_biased_integer(S):
+    """One rounded double addition yields m = N + B_S in its low 48 bits,
+    ties to even like np.rint (B_S is even), across the whole range."""
+    y = _samples(np.random.default_rng(10 + S), 4000)
+    # exact ties of the integer rounding: half-way points between integers
+    half = (np.arange(-40, 40) + 0.5) * 2.0 ** -(8 * S - 1)
+    y = np.concatenate([y, half])
+    m = (top_integer(y, S) + bias(S)) & ((1 << 48) - 1)
+    assert np.array_equal(magic_low_bits(y, S), m)
+    # planted: truncating instead of rounding differs on the ties
+    trunc = (np.trunc(np.ldexp(half, 8 * S - 1)).astype(np.int64) + bias(S)) & ((1 << 48) - 1)
+    assert not np.array_equal(magic_low_bits(half, S), trunc)
 
 
 def test_scale_keeps_the_top_digit_in_int8():
