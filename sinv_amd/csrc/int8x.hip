@@ -700,7 +700,8 @@ I8xPlan plan_i8x(const GemmArgs& g, int S) {
 }
 
 template <int S>
-hipError_t launch_i8x_s(const GemmArgs& g, const I8xPlan& p, uint8_t* base, hipStream_t s) {
+hipError_t launch_i8x_s(const GemmArgs& g, const I8xPlan& p, uint8_t* base, hipStream_t s,
+                        bool reuse_a) {
   // the launch profiler times the digit split and the product separately
   int pid = prof_open(s);
   // op(A): rows m, k along K;  element (m, k) = A[m*lda + k] (N) or A[k*lda + m] (T, C)
@@ -723,6 +724,7 @@ hipError_t launch_i8x_s(const GemmArgs& g, const I8xPlan& p, uint8_t* base, hipS
   const int nkc = p.Kp / kChunk;
   double* pmax = reinterpret_cast<double*>(base + p.pmax);
   for (const SliceArgs* a : {&sa, &sb}) {
+    if (a == &sa && reuse_a) continue;
     const unsigned nz = (unsigned)(a == &sa ? p.a_nz : p.b_nz);
     const bool kc_major = a->ks == 1;
     if (nkc <= kSplitMaxChunks) {
@@ -752,9 +754,10 @@ hipError_t launch_i8x_s(const GemmArgs& g, const I8xPlan& p, uint8_t* base, hipS
   {
     // split: each distinct operand copy read once (16 B per complex element),
     // its digits (3 S B per padded element) and scales written
-    const double rd = 16.0 * ((double)p.a_nz * g.M * g.K + (double)p.b_nz * g.N * g.K);
-    const double wr = 3.0 * S * ((double)p.a_nz * p.Mp + (double)p.b_nz * p.Np) * p.Kp +
-                      24.0 * ((double)p.a_nz * p.Mp + (double)p.b_nz * p.Np);
+    const double an = reuse_a ? 0.0 : (double)p.a_nz;
+    const double rd = 16.0 * (an * g.M * g.K + (double)p.b_nz * g.N * g.K);
+    const double wr = 3.0 * S * (an * p.Mp + (double)p.b_nz * p.Np) * p.Kp +
+                      24.0 * (an * p.Mp + (double)p.b_nz * p.Np);
     prof_close(pid, s, PROF_I8X_SPLIT, 0.0, rd + wr);
   }
   pid = prof_open(s);
@@ -792,7 +795,7 @@ size_t gemm_i8x_workspace_bytes(const GemmArgs& g, int S) {
 }
 
 hipError_t launch_gemm_i8x(const GemmArgs& g, int S, void* work, size_t work_bytes,
-                           hipStream_t s) {
+                           hipStream_t s, bool reuse_a) {
   if (g.M <= 0 || g.N <= 0 || g.inner <= 0 || g.outer <= 0) return hipSuccess;
   if (g.K <= 0) return launch_gemm(g, Prec::F64, s);  // no product term: C = beta C0 + shift
   if (!i8x_slices_ok(S)) return hipErrorInvalidValue;
@@ -804,9 +807,9 @@ hipError_t launch_gemm_i8x(const GemmArgs& g, int S, void* work, size_t work_byt
     return hipErrorInvalidConfiguration;
   uint8_t* base = (uint8_t*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
   switch (S) {
-    case 5: return launch_i8x_s<5>(g, p, base, s);
-    case 6: return launch_i8x_s<6>(g, p, base, s);
-    default: return launch_i8x_s<7>(g, p, base, s);
+    case 5: return launch_i8x_s<5>(g, p, base, s, reuse_a);
+    case 6: return launch_i8x_s<6>(g, p, base, s, reuse_a);
+    default: return launch_i8x_s<7>(g, p, base, s, reuse_a);
   }
 }
 

@@ -166,7 +166,7 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
   auto gemm = [&](int M, int Nn, int K, const cplx* A, int lda, long long sA, const cplx* B,
                   int ldb, long long sB, cplx* C, int ldc, long long sC, cplx alpha,
                   const cplx* C0, int ldc0, long long sC0, cplx beta, const cplx* sh,
-                  Prec p) -> hipError_t {
+                  Prec p, bool reuse_a = false) -> hipError_t {
     GemmArgs g{};
     g.M = M; g.N = Nn; g.K = K; g.opA = OP_N; g.opB = OP_N;
     g.A = A; g.lda = lda; g.sA_in = 0; g.sA_out = sA;
@@ -175,7 +175,7 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
     g.C = C; g.ldc = ldc; g.sC_in = 0; g.sC_out = sC;
     g.alpha = alpha; g.beta = beta; g.shift = sh;
     g.inner = 1; g.outer = nb;
-    if (i8x) return launch_gemm_i8x(g, kI8xDefaultSlices, wi8x, wp.i8x_bytes, s);
+    if (i8x) return launch_gemm_i8x(g, kI8xDefaultSlices, wi8x, wp.i8x_bytes, s, reuse_a);
     return launch_gemm(g, p, s);
   };
 
@@ -282,9 +282,11 @@ extern "C" int sinv_rgf_batched(const sinv_rgf_desc* d, void* work, size_t work_
                           bprec));
       }
     }
-    // G_ii = g_i + Q X_i = g_i - Q G_{i+1,i}   (in place over g_i)
+    // G_ii = g_i + Q X_i = g_i - Q G_{i+1,i}   (in place over g_i); in int8x
+    // mode Q's digits are those of the G_{i,i+1} product just before
+    const bool q_digits_ready = save && !d->sym_mat;
     SINV_HIP_TRY(gemm(bi, bi, bj, Q, bj, b2, Xn, bi, sX, Gd + L.doff[i], bi, L.dsize, mone,
-                      Gd + L.doff[i], bi, L.dsize, one, nullptr, bprec));
+                      Gd + L.doff[i], bi, L.dsize, one, nullptr, bprec, q_digits_ready));
   }
   if (!save && N > 1) {
     // rgf.py:41,55: unsaved off-diagonal blocks of G stay NaN

@@ -54,12 +54,14 @@ hipError_t launch_gemm(const GemmArgs& g, Prec p, hipStream_t s);
 // K2 in FP64-accurate sliced-int8 form (int8x.hip): the same product from S
 // balanced base-256 digit planes per operand on the int8 matrix cores, exact
 // int32 sums.  The workspace holds the digit planes and row scales of both
-// operands.
+// operands.  reuse_a: op(A)'s digits and scales are already in the workspace
+// (the previous call on this workspace and stream had the same op(A), M, K and
+// batch layout), so its split is skipped.
 constexpr int kI8xMinSlices = 5, kI8xMaxSlices = 7, kI8xDefaultSlices = 6;
 bool i8x_slices_ok(int S);
 size_t gemm_i8x_workspace_bytes(const GemmArgs& g, int S);
 hipError_t launch_gemm_i8x(const GemmArgs& g, int S, void* work, size_t work_bytes,
-                           hipStream_t s);
+                           hipStream_t s, bool reuse_a = false);
 
 // C[z] = a * D[z] + shift[z_outer] * I  (elementwise, b x b blocks), batched as GemmArgs.
 hipError_t launch_axpby_shift(int M, int N, cplx a, const cplx* D, int ldd, long long sD_in,
