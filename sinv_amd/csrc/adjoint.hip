@@ -168,8 +168,11 @@ extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* wo
   };
   auto ws = [&](const cplx* p, int ld) { return Opd{p, WI, OP_N, ld}; };
   // C (M x N, ldc) = alpha op(A) op(B) (+ C0 when given), contraction length K
+  // reuse_a (int8x): op(A) is the previous product's op(A), whose digit planes
+  // are still in the workspace
   auto gemm = [&](int M, int Nn, int K, Opd A, Opd B, cplx* C, long long cinst, int ldc,
-                  cplx alpha, const cplx* C0, long long c0inst) -> hipError_t {
+                  cplx alpha, const cplx* C0, long long c0inst,
+                  bool reuse_a = false) -> hipError_t {
     GemmArgs g{};
     g.M = M; g.N = Nn; g.K = K; g.opA = A.op; g.opB = B.op;
     g.A = A.p; g.lda = A.ld; g.sA_in = 0; g.sA_out = A.inst;
@@ -180,11 +183,13 @@ extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* wo
     g.beta = C0 ? one : zero;
     g.shift = nullptr;
     g.inner = 1; g.outer = nb;
-    if (i8x) return launch_gemm_i8x(g, kI8xDefaultSlices, wi8x, P.i8x_bytes, s);
+    if (i8x) return launch_gemm_i8x(g, kI8xDefaultSlices, wi8x, P.i8x_bytes, s, reuse_a);
     return launch_gemm(g, prec, s);
   };
   auto mm = [&](int M, int Nn, int K, Opd A, Opd B, cplx* C, long long cinst, int ldc, cplx alpha,
-                bool acc) { return gemm(M, Nn, K, A, B, C, cinst, ldc, alpha, acc ? C : nullptr, cinst); };
+                bool acc, bool reuse_a = false) {
+    return gemm(M, Nn, K, A, B, C, cinst, ldc, alpha, acc ? C : nullptr, cinst, reuse_a);
+  };
   auto copy = [&](int r, int c, const cplx* src, long long sinst, cplx* dst, long long dinst) {
     return launch_axpby_shift(r, c, one, src, c, 0, sinst, dst, c, 0, dinst, nullptr, 1, nb, s);
   };
@@ -220,16 +225,18 @@ extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* wo
     const int bp = bs(i - 1);
     cplx* Y = slot_y(i);
     cplx* Qf = slot_qf(i);
-    // Y = B_{i,i-1} g_{i-1}  (b_i x b_{i-1});  M = B_ii - Y B_{i-1,i}
+    // Y = B_{i,i-1} g_{i-1}  (b_i x b_{i-1});  Qf = B_{i,i-1} dg_{i-1} + T_{i,i-1} g_{i-1}
+    // (products with the same left operand back to back: in int8x mode its
+    // digit planes are split once)
     SINV_HIP_RET(mm(bi, bp, bp, Ublk(i - 1), ws(gp, bp), Y, WI, bp, cos_, false));
+    SINV_HIP_RET(mm(bi, bp, bp, Ublk(i - 1), ws(dgp, bp), Qf, WI, bp, cos_, false, true));
+    SINV_HIP_RET(mm(bi, bp, bp, Tlb(i - 1), ws(gp, bp), Qf, WI, bp, one, true));
+    // M = B_ii - Y B_{i-1,i};  dM = T_ii - Y T_{i-1,i} - Qf B_{i-1,i}
     SINV_HIP_RET(
         gemm(bi, bi, bp, ws(Y, bp), Lblk(i - 1), oop ? Bd : go, WI, bi, mcos, Bd, WI));
-    // dM = T_ii - (T_{i,i-1} g_{i-1} + B_{i,i-1} dg_{i-1}) B_{i-1,i} - Y T_{i-1,i}
-    SINV_HIP_RET(mm(bi, bp, bp, Tlb(i - 1), ws(gp, bp), Qf, WI, bp, one, false));
-    SINV_HIP_RET(mm(bi, bp, bp, Ublk(i - 1), ws(dgp, bp), Qf, WI, bp, cos_, true));
-    SINV_HIP_RET(gemm(bi, bi, bp, ws(Qf, bp), Lblk(i - 1), dM, WI, bi, mcos, Td + Ly.doff[i],
-                      d->gbar_diag_stride));
-    SINV_HIP_RET(mm(bi, bi, bp, ws(Y, bp), Tub(i - 1), dM, WI, bi, mone, true));
+    SINV_HIP_RET(gemm(bi, bi, bp, ws(Y, bp), Tub(i - 1), dM, WI, bi, mone, Td + Ly.doff[i],
+                      d->gbar_diag_stride, true));
+    SINV_HIP_RET(mm(bi, bi, bp, ws(Qf, bp), Lblk(i - 1), dM, WI, bi, mcos, true));
     // g_i = M^-1, dg_i = -g_i dM g_i
     SINV_HIP_RET(invert(i));
     SINV_HIP_RET(mm(bi, bi, bi, ws(go, bi), ws(dM, bi), Wt, WI, bi, one, false));
@@ -280,21 +287,24 @@ extern "C" int sinv_rgf_adjoint_batched(const sinv_rgf_adjoint_desc* d, void* wo
       cplx* Adi = Ad + Ly.doff[i];  // b_i x b_i
       // the forward step i+1's Y = B_{i+1,i} g and Qf = T_{i+1,i} g + B_{i+1,i} dg
       const Opd Yn = ws(slot_y(i + 1), bi), Qn = ws(slot_qf(i + 1), bi);
-      // primal: Xn = -Gn Y; Q = g B_{i,i+1}; G_ii = g - Q Xn
+      // Products with the same left operand run back to back (in int8x mode
+      // its digit planes are split once: Gn, g, Q and S each once).
+      // Xn = -Gn Y;  tangent of G_{i+1,i} = -Gn B_{i+1,i} g:  -Gn Qf - dGn Y
       SINV_HIP_TRY(mm(bj, bi, bj, ws(Gn, bj), Yn, Xn, WI, bi, mone, false));
+      SINV_HIP_TRY(mm(bj, bi, bj, ws(Gn, bj), Qn, Ali, AoI, bi, mone, false, true));
+      SINV_HIP_TRY(mm(bj, bi, bj, dGn, Yn, Ali, AoI, bi, mone, true));
+      // Q = g B_{i,i+1};  S = g T_{i,i+1} + dg B_{i,i+1}
       SINV_HIP_TRY(mm(bi, bj, bi, ws(g, bi), Lblk(i), Qb, WI, bj, cos_, false));
+      SINV_HIP_TRY(mm(bi, bj, bi, ws(g, bi), Tub(i), Sb, WI, bj, one, false, true));
+      SINV_HIP_TRY(mm(bi, bj, bi, ws(dg, bi), Lblk(i), Sb, WI, bj, cos_, true));
+      // G_ii = g - Q Xn;  tangent of G_{i,i+1}: -Q dGn - S Gn;
+      // tangent of G_ii: dg - Q dXn - S Xn
       SINV_HIP_TRY(gemm(bi, bi, bj, ws(Qb, bj), ws(Xn, bi), Gi, WI, bi, mone, g, WI));
-      // tangent of G_{i+1,i} = -Gn B_{i+1,i} g:  -dGn Y - Gn Qf
-      SINV_HIP_TRY(mm(bj, bi, bj, dGn, Yn, Ali, AoI, bi, mone, false));
-      SINV_HIP_TRY(mm(bj, bi, bj, ws(Gn, bj), Qn, Ali, AoI, bi, mone, true));
-      // tangent of G_{i,i+1}: -(dg B_{i,i+1} + g T_{i,i+1}) Gn - Q dGn
-      SINV_HIP_TRY(mm(bi, bj, bi, ws(dg, bi), Lblk(i), Sb, WI, bj, cos_, false));
-      SINV_HIP_TRY(mm(bi, bj, bi, ws(g, bi), Tub(i), Sb, WI, bj, one, true));
-      SINV_HIP_TRY(mm(bi, bj, bj, ws(Sb, bj), ws(Gn, bj), Aui, AoI, bj, mone, false));
-      SINV_HIP_TRY(mm(bi, bj, bj, ws(Qb, bj), dGn, Aui, AoI, bj, mone, true));
-      // tangent of G_ii: dg - S Xn - Q dXn
-      SINV_HIP_TRY(gemm(bi, bi, bj, ws(Sb, bj), ws(Xn, bi), Adi, AdI, bi, mone, dg, WI));
-      SINV_HIP_TRY(mm(bi, bi, bj, ws(Qb, bj), Opd{Ali, AoI, OP_N, bi}, Adi, AdI, bi, mone, true));
+      SINV_HIP_TRY(mm(bi, bj, bj, ws(Qb, bj), dGn, Aui, AoI, bj, mone, false, true));
+      SINV_HIP_TRY(gemm(bi, bi, bj, ws(Qb, bj), Opd{Ali, AoI, OP_N, bi}, Adi, AdI, bi, mone, dg,
+                        WI, true));
+      SINV_HIP_TRY(mm(bi, bj, bj, ws(Sb, bj), ws(Gn, bj), Aui, AoI, bj, mone, true));
+      SINV_HIP_TRY(mm(bi, bi, bj, ws(Sb, bj), ws(Xn, bi), Adi, AdI, bi, mone, true, true));
       if (Gout != nullptr)
         SINV_HIP_TRY(launch_transpose(bi, bi, OP_C, Gi, bi, 0, WI, Gout + Ly.doff[i], bi, 0, AdI,
                                       1, nb, s));
