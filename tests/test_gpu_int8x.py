@@ -147,6 +147,40 @@ def test_rgf_int8x_energies_meet_the_gate(built_lib):
         assert dense.rel_err(got, ref) < 1e-8
 
 
+def test_rgf_int8x_symmetric_and_plane_reuse(built_lib):
+    """The G_ii product reuses the digit planes of Q = g_i A_{i,i+1} from the
+    G_{i,i+1} product just before it; with sym_mat there is no such product
+    and Q is split for G_ii itself.  Both give FP64's blocks to 1e-10 (a
+    symmetric instance: A = A^T), and the launch profiler's split bytes equal
+    the count with exactly one Q split per backward step either way."""
+    from sinv_amd import core, _native as nat
+    N, b, nb = 5, 128, 2
+    rng = np.random.default_rng(17)
+    d = rng.standard_normal((N, b, b)) + 1j * rng.standard_normal((N, b, b))
+    d = d + d.transpose(0, 2, 1) + 40 * np.eye(b)
+    u = rng.standard_normal((N - 1, b, b)) + 1j * rng.standard_normal((N - 1, b, b))
+    l = u.transpose(0, 2, 1).copy()
+    sh = np.array([0.5 + 1e-2j, -1.0 + 1e-2j])
+    print("rgf int8x sym / reuse N=5 b=128", flush=True)
+    split = {}
+    for sym in (False, True):
+        G64 = core.rgf_batched(d, u, l, shift=sh, sym_mat=sym)
+        with nat.LaunchProfiler() as p:
+            G8 = core.rgf_batched(d, u, l, shift=sh, sym_mat=sym, precision="int8x")
+        split[sym] = p.bytes["i8x_split"]
+        for x, y in zip(G8, G64):
+            x, y = x.cpu().numpy(), y.cpu().numpy()
+            assert np.linalg.norm(x - y) <= 1e-10 * np.linalg.norm(y), sym
+    # one split of a distinct b x b copy: 16 B read per element, 18 B of digits
+    # written per element, 24 B of scales per row
+    per_copy = 16 * b * b + 18 * b * b + 24 * b
+    fwd = (N - 1) * ((1 + nb) + (nb + 1))          # L g (L shared), T U (U shared)
+    bwd_saved = (N - 1) * (2 * nb + (nb + 1) + 2 * nb + nb)   # Xn, Q, G_{i,i+1}, G_ii (Q reused)
+    bwd_sym = (N - 1) * (2 * nb + (nb + 1) + 2 * nb)          # Xn, Q, G_ii (G_{i,i+1} = G_{i+1,i}^T)
+    assert split[False] == per_copy * (fwd + bwd_saved)
+    assert split[True] == per_copy * (fwd + bwd_sym)
+
+
 def test_rgf_int8x_ragged_and_unsaved(built_lib):
     """Per-block sizes (not multiples of 64) and save_off_diag=False in int8x mode."""
     from sinv_amd import core
