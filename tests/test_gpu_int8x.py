@@ -181,6 +181,25 @@ def test_rgf_int8x_symmetric_and_plane_reuse(built_lib):
     assert split[True] == per_copy * (fwd + bwd_sym)
 
 
+def test_rgf_int8x_blocks_above_the_single_pass_split(built_lib):
+    """Blocks of 576 (K > 512: the row-maximum pass + digit pass split) and a
+    ragged neighbour of 448 (single pass) in one RGF solve: int8x within 1e-10
+    of FP64 on a well-conditioned instance."""
+    from sinv_amd import core
+    from sinv_amd.utils import configs
+    print("rgf int8x b=576/448", flush=True)
+    bs = [576, 448, 576]
+    H = configs.hermitian_H_ragged(bs, seed=41)
+    d, u, l = configs.shifted_lists(H, 0.2 + 5e-2j)
+    G64 = core.rgf_ragged(d, u, l)
+    G8 = core.rgf_ragged(d, u, l, precision="int8x")
+    for x, y in zip(G8, G64):
+        for a, b in zip(x, y):
+            a = a.cpu().numpy() if hasattr(a, "cpu") else a
+            b = b.cpu().numpy() if hasattr(b, "cpu") else b
+            assert np.linalg.norm(a - b) <= 1e-10 * np.linalg.norm(b)
+
+
 def test_rgf_int8x_ragged_and_unsaved(built_lib):
     """Per-block sizes (not multiples of 64) and save_off_diag=False in int8x mode."""
     from sinv_amd import core
