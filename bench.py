@@ -583,7 +583,7 @@ def main():
             "int8x": "gemm_i8x"}[head]
     kms, kcnt, kfl = results[head]["prof"][kind]
     kname = {"fp64": "gemm_kernel<double>", "mixed": "gemm_kernel<float>",
-             "mixed_refined": "gemm_kernel<double>", "int8x": "gemm_i8x_o2_kernel<6>"}[head]
+             "mixed_refined": "gemm_kernel<double>", "int8x": "gemm_i8x_o2_kernel<6, true>"}[head]
     tkey = "gemm_i8x_o2_kernel" if head == "int8x" else kname  # name in the traffic summary
     achieved = kfl / (kms / 1e3) / 1e12 if kms > 0 else 0.0
     if head == "int8x":
@@ -850,7 +850,7 @@ def adjoint_record(args, modes=("fp64", "mixed", "int8x"), steps=None, warmup=No
     achieved = kfl / (kms / 1e3) / 1e12 if kms > 0 else 0.0
     if head == "int8x":
         achieved *= I8X_OPS_PER_FLOP  # executed int8 operations (3 x 21 digit pairs x 2 MNK)
-        kname, tkey, exe = "gemm_i8x_o2_kernel<6>", "gemm_i8x_o2_kernel", 1.0
+        kname, tkey, exe = "gemm_i8x_o2_kernel<6, true>", "gemm_i8x_o2_kernel", 1.0
     else:
         # FP64 complex products run as 3 real MFMA products (3M): 3/4 of 8MNK executed
         kname, tkey, exe = "gemm_kernel<double>", "gemm_kernel<double>", 0.75

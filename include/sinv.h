@@ -407,8 +407,9 @@ int sinv_zgemm_int8x_batched(int M, int N, int K, int opA, int opB, sinv_c128 al
 
 /* Product kernel of the int8x GEMM (no counterpart in the reference): 0 =
  * two LDS buffers, one workgroup per CU; 1 = one LDS buffer, two workgroups
- * per CU (default); -1 restores the default.  Both give identical bits (the
- * digit sums are exact and the fold order is the same).  Process-wide. */
+ * per CU, 80-byte padded LDS rows; 2 = the same with swizzled 64-byte rows
+ * (default); -1 restores the default.  All give identical bits (the digit
+ * sums are exact and the fold order is the same).  Process-wide. */
 int sinv_int8x_force_variant(int variant);
 
 /* ------------------------------------------------------------------------

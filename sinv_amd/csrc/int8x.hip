@@ -525,14 +525,20 @@ __global__ __launch_bounds__(256) void gemm_i8x_kernel(GemmArgs g, ProdArgs o, i
       }
 }
 
-// Same product with a single LDS buffer (S x 64 x 80 bytes per operand, so
-// two workgroups share a CU) and registers capped for two waves per SIMD:
-// each chunk is loaded, stored to LDS between two barriers and then
-// multiplied; the other workgroup's MFMAs cover this one's load latency.
-template <int S>
+// Same product with a single LDS buffer (S x 64 rows per operand, so two
+// workgroups share a CU) and registers capped for two waves per SIMD: each
+// chunk is loaded, stored to LDS between two barriers and then multiplied;
+// the other workgroup's MFMAs cover this one's load latency.  The default
+// (variant 2) swizzles its 64-byte LDS rows, variant 1 pads them to 80 bytes.
+template <int S, bool SWZ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void gemm_i8x_o2_kernel(GemmArgs g, ProdArgs o, int tiles_m, int tiles_n) {
-  __shared__ __attribute__((aligned(16))) int8_t lds[2][S][kTile][kRowB];
+  // SWZ: 64-byte LDS rows whose four 16-byte pieces sit at slots q ^ f(r)
+  // (f(r) = ((r >> 1) ^ (r >> 3)) & 3), conflict-free for the stores (four
+  // lanes per row) and the fragment reads (sixteen rows per piece) alike;
+  // otherwise 80-byte rows (conflict-free reads only)
+  constexpr int kPitch = SWZ ? 64 : kRowB;
+  __shared__ __attribute__((aligned(16))) int8_t lds[2][S][kTile][kPitch];
 
   int bid = blockIdx.x;
   const int total = gridDim.x;
@@ -553,6 +559,10 @@ void gemm_i8x_o2_kernel(GemmArgs g, ProdArgs o, int tiles_m, int tiles_n) {
   const int8_t* a_src = o.a_data + (size_t)za * 3 * S * o.a_Rp * Kp + (size_t)(m0 + lr) * 64 + lc;
   const int8_t* b_src = o.b_data + (size_t)zb * 3 * S * o.b_Rp * Kp + (size_t)(n0 + lr) * 64 + lc;
   const int fr = lane & 15, fk = (lane >> 4) * 16;
+  auto swz = [](int r) { return ((r >> 1) ^ (r >> 3)) & 3; };  // of the row mod 16
+  // store slot of this thread's piece, fragment slot of this lane's piece
+  const int st = SWZ ? (((lc >> 4) ^ swz(lr & 15)) << 4) : lc;
+  const int fs = SWZ ? (((fk >> 4) ^ swz(fr)) << 4) : fk;
 
   double cr[2][2][4], ci[2][2][4];
 #pragma unroll
@@ -583,8 +593,8 @@ void gemm_i8x_o2_kernel(GemmArgs g, ProdArgs o, int tiles_m, int tiles_n) {
         __syncthreads();  // every wave is done with the previous chunk
 #pragma unroll
         for (int d = 0; d < S; ++d) {
-          *reinterpret_cast<i4v*>(&lds[0][d][lr][lc]) = ra[d];
-          *reinterpret_cast<i4v*>(&lds[1][d][lr][lc]) = rb[d];
+          *reinterpret_cast<i4v*>(&lds[0][d][lr][st]) = ra[d];
+          *reinterpret_cast<i4v*>(&lds[1][d][lr][st]) = rb[d];
         }
         __syncthreads();
       }
@@ -595,13 +605,13 @@ void gemm_i8x_o2_kernel(GemmArgs g, ProdArgs o, int tiles_m, int tiles_n) {
         i4v bf[S];
 #pragma unroll
         for (int u = 0; u < S; ++u)
-          bf[u] = *reinterpret_cast<const i4v*>(&lds[1][u][wn * 32 + ct * 16 + fr][fk]);
+          bf[u] = *reinterpret_cast<const i4v*>(&lds[1][u][wn * 32 + ct * 16 + fr][fs]);
 #pragma unroll
         for (int ta = 0; ta < S; ++ta)
 #pragma unroll
           for (int rt = 0; rt < 2; ++rt) {
             const i4v af =
-                *reinterpret_cast<const i4v*>(&lds[0][ta][wm * 32 + rt * 16 + fr][fk]);
+                *reinterpret_cast<const i4v*>(&lds[0][ta][wm * 32 + rt * 16 + fr][fs]);
 #pragma unroll
             for (int ub = 0; ta + ub < S; ++ub)
               acc[rt][ct][ta + ub] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
@@ -661,7 +671,7 @@ void gemm_i8x_o2_kernel(GemmArgs g, ProdArgs o, int tiles_m, int tiles_n) {
 }
 
 int g_i8x_variant = -1;  // forced product kernel (-1: default)
-constexpr int kI8xDefaultVariant = 1;
+constexpr int kI8xDefaultVariant = 2;
 
 struct I8xPlan {
   int a_mode, b_mode;
@@ -771,9 +781,12 @@ hipError_t launch_i8x_s(const GemmArgs& g, const I8xPlan& p, uint8_t* base, hipS
   const int variant = g_i8x_variant >= 0 ? g_i8x_variant : kI8xDefaultVariant;
   if (variant == 0)
     hipLaunchKernelGGL(gemm_i8x_kernel<S>, dim3((unsigned)blocks), dim3(256), 0, s, g, o, tm, tn);
+  else if (variant == 2)
+    hipLaunchKernelGGL((gemm_i8x_o2_kernel<S, true>), dim3((unsigned)blocks), dim3(256), 0, s, g, o,
+                       tm, tn);
   else
-    hipLaunchKernelGGL(gemm_i8x_o2_kernel<S>, dim3((unsigned)blocks), dim3(256), 0, s, g, o, tm,
-                       tn);
+    hipLaunchKernelGGL((gemm_i8x_o2_kernel<S, false>), dim3((unsigned)blocks), dim3(256), 0, s, g, o,
+                       tm, tn);
   SINV_HIP_RET(hipGetLastError());
   // product: the digit planes read once, C written (and C0 read) once
   const double nb = (double)g.inner * g.outer;
@@ -871,8 +884,8 @@ extern "C" int sinv_zgemm_int8x_batched(int M, int N, int K, int opA, int opB, s
 }
 
 extern "C" int sinv_int8x_force_variant(int variant) {
-  if (variant < -1 || variant > 1) {
-    set_error("sinv_int8x_force_variant: variant must be -1, 0 or 1");
+  if (variant < -1 || variant > 2) {
+    set_error("sinv_int8x_force_variant: variant must be -1, 0, 1 or 2");
     return SINV_E_INVAL;
   }
   sinv::g_i8x_variant = variant;

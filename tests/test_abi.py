@@ -147,7 +147,7 @@ def test_int8x_host_validation(built_lib):
                                     ctypes.c_void_p(16), 4, 0, nat.C128.of(0), None, 4, 0,
                                     ctypes.c_void_p(16), 4, 0, None, 1, 7, None, 0, None)
     assert rc == nat.SINV_E_INVAL and b"exceeds" in L.sinv_last_error()
-    assert L.sinv_int8x_force_variant(2) == nat.SINV_E_INVAL
+    assert L.sinv_int8x_force_variant(3) == nat.SINV_E_INVAL
     assert L.sinv_int8x_force_variant(-1) == nat.SINV_OK
     assert L.sinv_gj_force_variant(3) == nat.SINV_E_INVAL
     assert L.sinv_gj_force_variant(2) == nat.SINV_OK

@@ -222,23 +222,29 @@ def test_rgf_int8x_ragged_and_unsaved(built_lib):
 
 
 def test_zgemm_int8x_product_variants_identical(built_lib):
-    """Both product kernels (two LDS buffers / one) give the same bits."""
+    """Every product kernel (two LDS buffers, one workgroup per CU; one LDS
+    buffer with padded rows, two workgroups per CU; the same with swizzled
+    rows, the default) gives the same bits, for each digit-plane count."""
     import torch
     from sinv_amd import core, _native as nat
     rng = np.random.default_rng(21)
-    A = _rand(rng, (4, 200, 150))
-    B = _rand(rng, (1, 150, 130))
     L = nat.lib()
-    outs = []
-    try:
-        for v in (0, 1):
-            nat.check(L.sinv_int8x_force_variant(v), "force_variant")
-            outs.append(core.zgemm_batched(torch.from_numpy(A), torch.from_numpy(B),
-                                           precision="int8x").cpu().numpy())
-    finally:
-        nat.check(L.sinv_int8x_force_variant(-1), "force_variant")
-    assert np.array_equal(outs[0], outs[1])
-    assert L.sinv_int8x_force_variant(2) == nat.SINV_E_INVAL
+    for M, K, N, S in ((200, 150, 130, 6), (33, 700, 96, 6), (384, 384, 384, 6), (100, 96, 70, 5),
+                       (100, 96, 70, 7)):
+        print(f"int8x variants M={M} K={K} N={N} S={S}", flush=True)
+        A = _rand(rng, (3, M, K))
+        B = _rand(rng, (1, K, N))
+        outs = []
+        try:
+            for v in (0, 1, 2):
+                nat.check(L.sinv_int8x_force_variant(v), "force_variant")
+                outs.append(core.zgemm_batched(torch.from_numpy(A), torch.from_numpy(B),
+                                               precision="int8x", slices=S).cpu().numpy())
+        finally:
+            nat.check(L.sinv_int8x_force_variant(-1), "force_variant")
+        assert np.array_equal(outs[0], outs[1])
+        assert np.array_equal(outs[0], outs[2])
+    assert L.sinv_int8x_force_variant(3) == nat.SINV_E_INVAL
 
 
 def test_adjoint_int8x_against_dense_formula_and_fp64(built_lib):
